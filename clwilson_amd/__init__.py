@@ -1,0 +1,243 @@
+"""clwilson_amd -- MI355X Wilson-prime engine (Python host binding).
+
+A ctypes binding of libclwilson.so, whose C ABI (include/clwilson.h) replaces
+the reference's OpenCL layer (simpleCL.c). The compute runs in hand-written
+gfx950 HIP kernels; this module never computes anything itself and raises if
+the native library is missing or no GPU is present.
+
+Typical use:
+    import clwilson_amd as cw
+    with cw.Device(0) as dev:
+        res = dev.wilson_range(1_000_000, 2_000_000)
+        print(res.checksum_hex, res.results)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libclwilson.so")
+_lib = None
+
+CLW_OK = 0
+CLW_ERR_HIP = -1
+CLW_ERR_ARG = -2
+CLW_ERR_OVERFLOW = -3
+CLW_ERR_NOMEM = -4
+CLW_ERR_CHECK = -5
+CLW_ERR_STATE = -6
+CLW_ERR_IO = -7
+
+BLOCK_SPAN = 61440
+REC_CAP = 4096
+REC_WORDS = 8
+MAXP = 0xFFFFFFFFFFFFFFFF // 4
+
+
+class ClwError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("compute_units", C.c_int32),
+                ("clock_khz", C.c_int32), ("global_mem", C.c_uint64), ("max_alloc", C.c_uint64)]
+
+
+class TpTable(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("p", "type", "tp_target", "n_lo", "n_hi", "one_lo", "one_hi",
+                                          "r2_lo", "r2_hi", "ninv", "acc_lo", "acc_hi")] + \
+               [("count", C.c_uint32), ("limbs", C.c_uint32)]
+
+
+class SegBuffers(C.Structure):
+    _fields_ = [("rec", C.c_void_p), ("nrec", C.c_void_p), ("nprimes", C.c_void_p),
+                ("side", C.c_void_p * 3), ("side_count", C.c_void_p), ("flags", C.c_void_p),
+                ("total", C.c_void_p), ("max_blocks", C.c_uint32), ("side_cap", C.c_uint32)]
+
+
+class Options(C.Structure):
+    _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32)]
+
+
+class RangeStats(C.Structure):
+    _fields_ = [("pmin", C.c_uint64), ("pmax", C.c_uint64), ("checksum", C.c_uint64),
+                ("totalcount", C.c_uint64), ("typeTarget", C.c_uint64 * 3), ("maxtarget", C.c_uint64),
+                ("lastp", C.c_uint64), ("lastw", C.c_int64), ("tpcount", C.c_uint32),
+                ("tpcnt", C.c_uint32 * 3), ("resultcount", C.c_uint32), ("limbs", C.c_uint32),
+                ("seconds", C.c_double)]
+
+
+def lib():
+    """Loads libclwilson.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ClwError(CLW_ERR_STATE, f"{LIB_PATH} is missing: run python -m clwilson_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    P = C.POINTER
+    sig = {
+        "clw_init": (i32, [i32, P(vp)]),
+        "clw_shutdown": (i32, [vp]),
+        "clw_get_device_info": (i32, [vp, P(DeviceInfo)]),
+        "clw_last_error": (C.c_char_p, []),
+        "clw_alloc": (i32, [vp, C.c_size_t, P(vp)]),
+        "clw_free": (i32, [vp, vp]),
+        "clw_h2d": (i32, [vp, vp, vp, C.c_size_t, i32]),
+        "clw_d2h": (i32, [vp, vp, vp, C.c_size_t, i32]),
+        "clw_memset": (i32, [vp, vp, i32, C.c_size_t]),
+        "clw_finish": (i32, [vp]),
+        "clw_event_create": (i32, [vp, P(vp)]),
+        "clw_event_record": (i32, [vp, vp]),
+        "clw_event_query": (i32, [vp]),
+        "clw_event_wait": (i32, [vp]),
+        "clw_event_elapsed_ms": (i32, [vp, vp, P(C.c_float)]),
+        "clw_event_destroy": (i32, [vp]),
+        "clw_limbs_for": (i32, [u64]),
+        "clw_tp_table_alloc": (i32, [vp, u32, u32, P(TpTable)]),
+        "clw_tp_table_free": (i32, [vp, P(TpTable)]),
+        "clw_setup": (i32, [vp, P(TpTable), i32]),
+        "clw_seg_alloc": (i32, [vp, u32, u32, P(SegBuffers)]),
+        "clw_seg_free": (i32, [vp, P(SegBuffers)]),
+        "clw_seg_clear": (i32, [vp, P(SegBuffers)]),
+        "clw_sieve_segment": (i32, [vp, P(SegBuffers), u64, u64, P(u64), P(u32)]),
+        "clw_blocks_upto": (u32, [u64, u64, u64]),
+        "clw_mul_records": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, u32, vp, vp, u32]),
+        "clw_mul_side": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, vp, vp, u32]),
+        "clw_reduce": (i32, [vp, P(TpTable), u32, u32, u32, vp, vp, u32]),
+        "clw_iterate": (i32, [vp, P(TpTable), P(u64)]),
+        "clw_finalize": (i32, [vp, P(TpTable), vp, vp, vp]),
+        "clw_wilson_range": (i32, [vp, u64, u64, P(Options), P(RangeStats), vp, vp, vp, u64]),
+        "clw_format_result": (i32, [u64, u64, C.c_char_p, C.c_size_t]),
+        "clw_range_types": (i32, [u64, u64, P(u32), P(u64)]),
+        "clw_host_primes": (u64, [u64, u64, vp, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    """Names of every function declared in include/clwilson.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(_PKG), "include", "clwilson.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(clw_[a-z0-9_]+)\s*\(", txt)))
+
+
+def last_error() -> str:
+    return lib().clw_last_error().decode(errors="replace")
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != CLW_OK:
+        raise ClwError(rc, f"{what}: {last_error()}" if what else last_error())
+
+
+def host_primes(lo: int, hi: int) -> np.ndarray:
+    L = lib()
+    n = L.clw_host_primes(lo, hi, None, 0)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    L.clw_host_primes(lo, hi, out.ctypes.data, n)
+    return out[:n]
+
+
+def format_result(p: int, quot: int) -> str | None:
+    buf = C.create_string_buffer(128)
+    return buf.value.decode() if lib().clw_format_result(p, quot, buf, 128) else None
+
+
+def range_types(pmin: int, pmax: int) -> tuple[list[int], list[int]]:
+    cnt = (C.c_uint32 * 3)()
+    tg = (C.c_uint64 * 3)()
+    check(lib().clw_range_types(pmin, pmax, cnt, tg), "range_types")
+    return list(cnt), list(tg)
+
+
+@dataclass
+class RangeResult:
+    pmin: int
+    pmax: int
+    p: np.ndarray
+    quot: np.ndarray
+    rem: np.ndarray
+    checksum: int
+    totalcount: int
+    type_targets: list
+    tpcnt: list
+    resultcount: int
+    lastp: int
+    lastw: int
+    limbs: int
+    seconds: float
+    results: list = field(default_factory=list)
+
+    @property
+    def checksum_hex(self) -> str:
+        return f"{self.checksum:016X}"
+
+    def results_txt(self) -> str:
+        """results.txt contents (cl_wilson.cpp:657-688, 1093-1143)."""
+        body = "".join(self.results)
+        return body + ("no results\n" if not self.results else "") + f"{self.checksum:016X}\n"
+
+
+class Device:
+    """One GPU context (clw_init / clw_shutdown)."""
+
+    def __init__(self, index: int = 0):
+        self.ctx = C.c_void_p()
+        check(lib().clw_init(index, C.byref(self.ctx)), "clw_init")
+
+    def close(self) -> None:
+        if self.ctx:
+            lib().clw_shutdown(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        d = DeviceInfo()
+        check(lib().clw_get_device_info(self.ctx, C.byref(d)), "device_info")
+        return dict(name=d.name.decode(), arch=d.arch.decode(), compute_units=d.compute_units,
+                    clock_khz=d.clock_khz, global_mem=d.global_mem)
+
+    def wilson_range(self, pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
+                     verbose: bool = False, allow_check_failure: bool = False) -> RangeResult:
+        n = int(lib().clw_host_primes(pmin, pmax, None, 0))
+        p = np.zeros(max(n, 1), dtype=np.uint64)
+        q = np.zeros(max(n, 1), dtype=np.uint64)
+        r = np.zeros(max(n, 1), dtype=np.uint64)
+        st = RangeStats()
+        opt = Options(segment_span, target_waves, 1 if verbose else 0)
+        rc = lib().clw_wilson_range(self.ctx, pmin, pmax, C.byref(opt), C.byref(st), p.ctypes.data,
+                                    q.ctypes.data, r.ctypes.data, n)
+        if rc != CLW_OK and not (allow_check_failure and rc == CLW_ERR_CHECK):
+            check(rc, f"wilson_range [{pmin}, {pmax})")
+        res = RangeResult(pmin, pmax, p[:n], q[:n], r[:n], st.checksum, st.totalcount, list(st.typeTarget),
+                          list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs, st.seconds)
+        for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
+            line = format_result(pi, qi)
+            if line:
+                res.results.append(line)
+        return res
+
+
+def wilson_quotient(p: int, quot: int) -> int:
+    """Signed w_p as the result lines print it (cl_wilson.cpp:872-898)."""
+    neg = p - quot
+    return quot if quot <= neg else -neg

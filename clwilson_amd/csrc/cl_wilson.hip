@@ -1,0 +1,456 @@
+// cl_wilson.cpp -- host orchestration of one Wilson-prime range on the GPU
+// (the role of cl_wilson() in the reference, cl_wilson.cpp:1442-1974).
+//
+// Steps: exact test primes of [pmin, pmax) on the host; typing and type
+// targets; per-prime constants on the device; a loop over segments of the
+// sieving-prime axis [7, maxtarget] (sieve + products + reduction per type);
+// the iterate step to each prime's own target; the batched final check; the
+// checksum and result classification on the host.
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "clw_internal.h"
+#include "clw_nt64.h"
+
+namespace {
+
+const uint64_t kMaxP = 0xFFFFFFFFFFFFFFFFull / 4;  // cl_wilson.h:9
+const uint64_t kMaxWidth = 10000000;               // cl_wilson.cpp:1162-1166
+const uint64_t kDefaultSpan = 4370ull * CLW_BLOCK_SPAN;
+const uint32_t kSideCap = 1u << 22;
+const uint32_t kMaxKRec = 32, kMaxKSide = 8, kMaxKLanes = 8192;
+const uint32_t kLanesModeBelow = 8192;              // test primes per type
+const uint64_t kLanesSpan = 16ull * kDefaultSpan;   // ~4.3e9 integers per segment
+
+// Odd primes below `lim` (simple sieve; lim is at most a few million).
+std::vector<uint32_t> small_odd_primes(uint32_t lim) {
+  std::vector<uint8_t> comp(lim + 1, 0);
+  std::vector<uint32_t> out;
+  for (uint64_t i = 3; i <= lim; i += 2) {
+    if (comp[i]) continue;
+    out.push_back((uint32_t)i);
+    for (uint64_t j = i * i; j <= lim; j += 2 * i) comp[j] = 1;
+  }
+  return out;
+}
+
+bool is_prime_det(uint64_t n) {
+  if (n < 2) return false;
+  static const uint32_t sp[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (uint32_t q : sp) {
+    if (n == q) return true;
+    if (n % q == 0) return false;
+  }
+  const clw::M64 m = clw::m64_make(n);
+  const int s = __builtin_ctzll(n - 1);
+  const uint64_t d = (n - 1) >> s;
+  if (!clw::sprp2(m, d, s)) return false;
+  return clw::mr_extra(m, d, s, clw::mr_bases_needed(n));
+}
+
+int64_t signed_w(uint64_t p, uint64_t quot) {
+  const uint64_t neg = p - quot;
+  return quot <= neg ? (int64_t)quot : -(int64_t)neg;
+}
+
+}  // namespace
+
+// Segmented sieve over [lo, hi) with trial primes up to min(sqrt(hi), 2^22);
+// when sqrt(hi) exceeds that, survivors are confirmed by deterministic
+// Miller-Rabin. Replaces primesieve_generate_primes (cl_wilson.cpp:1529).
+extern "C" uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap) {
+  if (hi <= lo) return 0;
+  uint64_t cnt = 0;
+  auto emit = [&](uint64_t v) {
+    if (out && cnt < cap) out[cnt] = v;
+    ++cnt;
+  };
+  if (lo <= 2 && hi > 2) emit(2);
+  uint64_t s = lo < 3 ? 3 : (lo | 1);
+  if (s >= hi) return cnt;
+  uint64_t r = (uint64_t)clw::isqrt64(hi) + 1;
+  const uint32_t lim = (uint32_t)std::min<uint64_t>(r, 1u << 22);
+  const bool need_mr = r > lim;
+  const std::vector<uint32_t> base = small_odd_primes(lim);
+  const uint64_t SEG = 1u << 20;  // odd numbers per window
+  std::vector<uint8_t> mark(SEG);
+  for (uint64_t a = s; a < hi; a += 2 * SEG) {
+    const uint64_t b = std::min<uint64_t>(a + 2 * SEG, hi);
+    const uint64_t n = (b - a + 1) / 2;
+    std::fill(mark.begin(), mark.begin() + n, 0);
+    for (uint32_t q32 : base) {
+      const uint64_t q = q32;
+      if (q * q >= b) break;
+      uint64_t st = q * q;
+      if (st < a) {
+        st = (a + q - 1) / q * q;
+        if ((st & 1) == 0) st += q;
+      }
+      for (uint64_t m = st; m < b; m += 2 * q) mark[(m - a) >> 1] = 1;
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t v = a + 2 * i;
+      if (v >= b) break;
+      if (mark[i] || v < 3) continue;
+      if (need_mr && !is_prime_det(v)) continue;
+      emit(v);
+    }
+  }
+  return cnt;
+}
+
+// cl_wilson.cpp:1552-1586: type 0 if p = 1 (mod 3) with target (p-1)/6,
+// type 1 if p = 5 (mod 12) with (p-1)/4, type 2 if p = 11 (mod 12) with
+// (p-1)/2; the type target is the target of the type's first prime.
+static inline uint32_t prime_type(uint64_t p) { return p % 3 == 1 ? 0 : (p % 12 == 5 ? 1 : 2); }
+static inline uint64_t type_divisor(uint32_t t) { return t == 0 ? 6 : (t == 1 ? 4 : 2); }
+
+extern "C" int clw_range_types(uint64_t pmin, uint64_t pmax, uint32_t tpcnt[3], uint64_t targets[3]) {
+  if (!tpcnt || !targets || pmin < 5 || pmax <= pmin) return clw_fail(CLW_ERR_ARG, "clw_range_types: bad range");
+  std::vector<uint64_t> ps(clw_host_primes(pmin, pmax, nullptr, 0));
+  clw_host_primes(pmin, pmax, ps.data(), ps.size());
+  for (int t = 0; t < 3; ++t) tpcnt[t] = 0, targets[t] = 0;
+  for (uint64_t p : ps) {
+    const uint32_t t = prime_type(p);
+    if (!tpcnt[t]++) targets[t] = (p - 1) / type_divisor(t);
+  }
+  return CLW_OK;
+}
+
+// cl_wilson.cpp:657-688 (result lines) and 872-898 (classification).
+extern "C" int clw_format_result(uint64_t p, uint64_t quot, char* buf, size_t buflen) {
+  const uint64_t neg = p - quot, smallest = quot <= neg ? quot : neg;
+  if (quot == 0) {
+    if (buf) snprintf(buf, buflen, "%llu is a Wilson prime\n", (unsigned long long)p);
+    return 1;
+  }
+  if (smallest < 1000) {
+    const int32_t dq = (smallest == quot) ? (int32_t)quot : -((int32_t)neg);
+    if (buf) snprintf(buf, buflen, "%llu is a Near-Wilson prime %+d\n", (unsigned long long)p, dq);
+    return 1;
+  }
+  if (buf && buflen) buf[0] = 0;
+  return 0;
+}
+
+namespace {
+
+#define CK(x)                        \
+  do {                               \
+    const int rc_ = (x);             \
+    if (rc_ != CLW_OK) return rc_;   \
+  } while (0)
+
+}  // namespace
+
+// A range being searched: host test primes, the device table, segment
+// buffers and partial-product scratch (the state cl_wilson() keeps in
+// progData/searchData, cl_wilson.h:27-70).
+struct clw_session {
+  clw_ctx* ctx = nullptr;
+  clw_options opt{};
+  uint64_t pmin = 0, pmax = 0;
+  std::vector<uint64_t> tp;   // increasing
+  std::vector<uint32_t> pos;  // pos[k]: index of tp[k] in the type-grouped table
+  uint32_t cnt[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+  uint64_t T[3] = {0, 0, 0}, maxT = 0;
+  uint32_t stride = 0, target_waves = 0;
+  bool lanes_mode = false;
+  uint64_t span = 0;
+  clw_tp_table tab{};
+  clw_seg_buffers seg{};
+  uint64_t* part_lo = nullptr;
+  uint64_t* part_hi = nullptr;
+  uint64_t total_primes = 0;  // sieving primes applied so far
+  bool iterated = false;
+  double gpu_seconds = 0;
+
+  ~clw_session() {
+    clw_tp_table_free(ctx, &tab);
+    clw_seg_free(ctx, &seg);
+    clw_free(ctx, part_lo);
+    clw_free(ctx, part_hi);
+  }
+};
+
+extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
+                                  clw_session** out) {
+  if (!ctx || !out) return clw_fail(CLW_ERR_ARG, "clw_session_create: null argument");
+  *out = nullptr;
+  // cl_wilson.cpp:1146-1182 setupSearch
+  if (pmin < 5 || pmax > kMaxP || pmin >= pmax)
+    return clw_fail(CLW_ERR_ARG, "range [%llu, %llu) invalid: need 5 <= pmin < pmax <= %llu",
+                    (unsigned long long)pmin, (unsigned long long)pmax, (unsigned long long)kMaxP);
+  if (pmax - pmin > kMaxWidth)
+    return clw_fail(CLW_ERR_ARG, "range width %llu > 10000000", (unsigned long long)(pmax - pmin));
+  clw_session* S = new clw_session();
+  S->ctx = ctx;
+  S->opt = opt ? *opt : clw_options{0, 0, 0};
+  S->pmin = pmin;
+  S->pmax = pmax;
+  S->tp.resize(clw_host_primes(pmin, pmax, nullptr, 0));
+  clw_host_primes(pmin, pmax, S->tp.data(), S->tp.size());
+  if (S->tp.empty()) {
+    delete S;
+    return clw_fail(CLW_ERR_ARG, "there are no primes to test in this range");
+  }
+  const uint32_t ntp = (uint32_t)S->tp.size();
+  for (uint64_t p : S->tp) {
+    const uint32_t t = prime_type(p);
+    if (!S->cnt[t]++) S->T[t] = (p - 1) / type_divisor(t);
+  }
+  S->off[1] = S->cnt[0];
+  S->off[2] = S->cnt[0] + S->cnt[1];
+  std::vector<uint64_t> gp(ntp);
+  std::vector<uint32_t> gt(ntp);
+  S->pos.resize(ntp);
+  {
+    uint32_t fill[3] = {S->off[0], S->off[1], S->off[2]};
+    for (uint32_t k = 0; k < ntp; ++k) {
+      const uint32_t t = prime_type(S->tp[k]);
+      S->pos[k] = fill[t]++;
+      gp[S->pos[k]] = S->tp[k];
+      gt[S->pos[k]] = t;
+    }
+  }
+  S->maxT = std::max(S->T[0], std::max(S->T[1], S->T[2]));
+  S->stride = std::max(S->cnt[0], std::max(S->cnt[1], S->cnt[2]));
+  // Few test primes: a wave per prime (clw_mul_records_lanes) and longer
+  // segments; many: a lane per prime (clw_mul_records).
+  S->lanes_mode = S->stride < kLanesModeBelow;
+  S->span = S->opt.segment_span
+                ? (S->opt.segment_span + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN * CLW_BLOCK_SPAN
+                : (S->lanes_mode ? kLanesSpan : kDefaultSpan);
+  S->target_waves = S->opt.target_waves ? S->opt.target_waves : (uint32_t)ctx->prop.multiProcessorCount * 4 * 8;
+  int rc = clw_tp_table_alloc(ctx, ntp, (uint32_t)clw_limbs_for(S->tp.back()), &S->tab);
+  if (rc == CLW_OK) rc = clw_h2d(ctx, S->tab.p, gp.data(), (size_t)ntp * 8, 1);
+  if (rc == CLW_OK) rc = clw_h2d(ctx, S->tab.type, gt.data(), (size_t)ntp * 4, 1);
+  if (rc == CLW_OK) rc = clw_setup(ctx, &S->tab, 0);  // setup.cl
+  if (rc == CLW_OK) rc = clw_seg_alloc(ctx, (uint32_t)(S->span / CLW_BLOCK_SPAN + 1), kSideCap, &S->seg);
+  // Chunk products of type t live at part[c * cnt[t] + i]: at most
+  // (kMaxKRec + kMaxKSide) * cnt (lane per prime) or target_waves + 2 cnt.
+  const size_t part_entries = std::max<size_t>((size_t)(kMaxKRec + kMaxKSide) * S->stride,
+                                               (size_t)S->target_waves + 2 * (size_t)S->stride);
+  if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_lo);
+  if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_hi);
+  if (rc == CLW_OK) rc = clw_finish(ctx);
+  if (rc != CLW_OK) {
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return CLW_OK;
+}
+
+extern "C" int clw_session_destroy(clw_session* S) {
+  delete S;
+  return CLW_OK;
+}
+
+extern "C" int clw_session_info(const clw_session* S, clw_range_stats* st) {
+  if (!S || !st) return clw_fail(CLW_ERR_ARG, "clw_session_info: null argument");
+  memset(st, 0, sizeof(*st));
+  st->pmin = S->pmin;
+  st->pmax = S->pmax;
+  st->tpcount = (uint32_t)S->tp.size();
+  for (int t = 0; t < 3; ++t) {
+    st->tpcnt[t] = S->cnt[t];
+    st->typeTarget[t] = S->T[t];
+  }
+  st->maxtarget = S->maxT;
+  st->totalcount = S->total_primes;
+  st->limbs = S->tab.limbs;
+  st->seconds = S->gpu_seconds;
+  return CLW_OK;
+}
+
+// Applies every sieving prime q in [qlo, qhi) (clipped to [2, maxtarget]) to
+// the residues: the segment loop of cl_wilson.cpp:1856-1929. Windows must not
+// overlap between calls; their union over a search is [2, maxtarget].
+extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
+  if (!S) return clw_fail(CLW_ERR_ARG, "clw_session_run: null session");
+  if (S->iterated) return clw_fail(CLW_ERR_STATE, "clw_session_run after iterate");
+  const auto t0 = std::chrono::steady_clock::now();
+  clw_ctx* ctx = S->ctx;
+  if (qhi > S->maxT + 1) qhi = S->maxT + 1;
+  // Primes 2, 3, 5 are outside the mod-30 wheel: their powers go to the side
+  // lists of the window that contains them.
+  std::vector<uint64_t> small_side[3];
+  for (uint32_t t = 0; t < 3; ++t)
+    for (uint64_t q : {2ull, 3ull, 5ull})
+      if (S->cnt[t] && q >= qlo && q < qhi && q <= S->T[t]) {
+        small_side[t].push_back(q);
+        small_side[t].push_back(clw::legendre_exp(q, S->T[t]));
+      }
+  for (uint64_t q : {2ull, 3ull, 5ull}) S->total_primes += (q >= qlo && q < qhi && q <= S->maxT);
+  const bool have_small = !small_side[0].empty() || !small_side[1].empty() || !small_side[2].empty();
+  uint64_t lo = std::max<uint64_t>(qlo, 7);
+  bool first = true;
+  auto last_report = std::chrono::steady_clock::now();
+  while ((first && have_small) || lo < qhi) {
+    const uint64_t hi = std::min<uint64_t>(lo + S->span - (lo % 30), qhi);
+    const bool have_range = lo < hi;
+    CK(clw_seg_clear(ctx, &S->seg));
+    if (first && have_small) {
+      uint32_t h_side_cnt[4] = {0, 0, 0, 0};
+      for (int t = 0; t < 3; ++t) {
+        if (small_side[t].empty()) continue;
+        h_side_cnt[t] = (uint32_t)(small_side[t].size() / 2);
+        CK(clw_h2d(ctx, S->seg.side[t], small_side[t].data(), small_side[t].size() * 8, 1));
+      }
+      CK(clw_h2d(ctx, S->seg.side_count, h_side_cnt, 16, 1));
+    }
+    if (have_range) {
+      uint32_t nb = 0;
+      CK(clw_sieve_segment(ctx, &S->seg, lo, hi, S->T, &nb));
+    }
+    for (uint32_t t = 0; t < 3; ++t) {
+      const uint32_t n = S->cnt[t];
+      if (!n) continue;
+      const bool recs = have_range && S->T[t] >= lo;
+      if (!recs && !(first && !small_side[t].empty())) continue;
+      const uint32_t nblk = recs ? clw_blocks_upto(lo, hi, S->T[t]) : 0;
+      uint32_t nparts = 0, Ks = 1;
+      if (S->lanes_mode) {
+        // a wave per test prime, lanes over the records; ~2+ blocks per chunk
+        uint32_t K = std::max<uint32_t>(1, (S->target_waves + n - 1) / n);
+        K = std::min<uint32_t>(K, std::max<uint32_t>(1, nblk / 2));
+        K = std::min<uint32_t>(K, kMaxKLanes);
+        if (nblk) {
+          CK(clw_mul_records_lanes(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n));
+          nparts = K;
+        }
+      } else {
+        const uint32_t tiles = (n + 255) / 256;
+        uint32_t K = std::max<uint32_t>(1, (S->target_waves + 4 * tiles - 1) / (4 * tiles));
+        K = std::min<uint32_t>(K, std::min<uint32_t>(kMaxKRec, std::max<uint32_t>(nblk, 1)));
+        Ks = std::min<uint32_t>(kMaxKSide, std::max<uint32_t>(1, K / 4));
+        if (nblk) {
+          CK(clw_mul_records(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n));
+          nparts = K;
+        }
+      }
+      CK(clw_mul_side(ctx, &S->tab, &S->seg, t, S->off[t], n, Ks, S->part_lo + (size_t)nparts * n,
+                      S->part_hi + (size_t)nparts * n, n));
+      nparts += Ks;
+      CK(clw_reduce(ctx, &S->tab, S->off[t], n, nparts, S->part_lo, S->part_hi, n));  // reduce.cl
+    }
+    // clearn.cl: add the segment's prime count; check the device flags.
+    uint32_t flags[4] = {0, 0, 0, 0};
+    uint64_t seg_total[2] = {0, 0};
+    CK(clw_d2h(ctx, flags, S->seg.flags, 16, 0));
+    CK(clw_d2h(ctx, seg_total, S->seg.total, 16, 1));
+    if (flags[0])
+      return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded in segment [%llu, %llu): flags %u",
+                      (unsigned long long)lo, (unsigned long long)hi, flags[0]);
+    S->total_primes += seg_total[0];
+    first = false;
+    if (!have_range) break;
+    lo = hi;
+    if (S->opt.verbose) {
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last_report).count() > 4.0) {
+        last_report = now;
+        fprintf(stderr, "%.4f%%\n", 100.0 * (double)lo / (double)(S->maxT + 1));
+      }
+    }
+  }
+  S->gpu_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return CLW_OK;
+}
+
+// Canonical residues (value mod p^2) in increasing p.
+extern "C" int clw_session_residues(clw_session* S, uint64_t* lo, uint64_t* hi, uint64_t cap) {
+  if (!S || !lo || !hi) return clw_fail(CLW_ERR_ARG, "clw_session_residues: null argument");
+  const uint32_t n = (uint32_t)S->tp.size();
+  std::vector<uint64_t> a(n), b(n);
+  CK(clw_d2h(S->ctx, a.data(), S->tab.acc_lo, (size_t)n * 8, 0));
+  CK(clw_d2h(S->ctx, b.data(), S->tab.acc_hi, (size_t)n * 8, 1));
+  for (uint32_t k = 0; k < n && k < cap; ++k) {
+    lo[k] = a[S->pos[k]];
+    hi[k] = b[S->pos[k]];
+  }
+  return CLW_OK;
+}
+
+extern "C" int clw_session_primes(const clw_session* S, uint64_t* out, uint64_t cap) {
+  if (!S || !out) return clw_fail(CLW_ERR_ARG, "clw_session_primes: null argument");
+  for (size_t k = 0; k < S->tp.size() && k < cap; ++k) out[k] = S->tp[k];
+  return CLW_OK;
+}
+
+// iterate.cl (cl_wilson.cpp:1938-1946).
+extern "C" int clw_session_iterate(clw_session* S) {
+  if (!S) return clw_fail(CLW_ERR_ARG, "clw_session_iterate: null session");
+  if (S->iterated) return clw_fail(CLW_ERR_STATE, "clw_session_iterate called twice");
+  const auto t0 = std::chrono::steady_clock::now();
+  CK(clw_iterate(S->ctx, &S->tab, S->T));
+  CK(clw_finish(S->ctx));
+  S->iterated = true;
+  S->gpu_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return CLW_OK;
+}
+
+// Final check (processResult, cl_wilson.cpp:691-900) for every test prime;
+// statistics as the reference accumulates them.
+extern "C" int clw_session_finalize(clw_session* S, clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot,
+                                    uint64_t* out_rem, uint64_t out_cap) {
+  if (!S || !stats) return clw_fail(CLW_ERR_ARG, "clw_session_finalize: null argument");
+  if (!S->iterated) return clw_fail(CLW_ERR_STATE, "clw_session_finalize before iterate");
+  const auto t0 = std::chrono::steady_clock::now();
+  clw_ctx* ctx = S->ctx;
+  const uint32_t ntp = (uint32_t)S->tp.size();
+  uint64_t *dq = nullptr, *dr = nullptr;
+  int rc = clw_alloc(ctx, (size_t)ntp * 8, (void**)&dq);
+  if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)ntp * 8, (void**)&dr);
+  std::vector<uint64_t> q(ntp), r(ntp);
+  if (rc == CLW_OK) rc = clw_finalize(ctx, &S->tab, dq, dr, nullptr);
+  if (rc == CLW_OK) rc = clw_d2h(ctx, q.data(), dq, (size_t)ntp * 8, 0);
+  if (rc == CLW_OK) rc = clw_d2h(ctx, r.data(), dr, (size_t)ntp * 8, 1);
+  clw_free(ctx, dq);
+  clw_free(ctx, dr);
+  if (rc != CLW_OK) return rc;
+  S->gpu_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  clw_session_info(S, stats);
+  int bad = 0;
+  uint64_t checksum = 0;
+  for (uint32_t k = 0; k < ntp; ++k) {
+    const uint64_t p = S->tp[k], quot = q[S->pos[k]], rem = r[S->pos[k]];
+    if (rem != 0) bad = 1;  // Wilson's theorem mod p (cl_wilson.cpp:866-870)
+    if (clw_format_result(p, quot, nullptr, 0)) {
+      stats->resultcount++;
+      stats->lastp = p;
+      stats->lastw = signed_w(p, quot);
+    }
+    checksum += p + rem + quot;
+    if (k < out_cap) {
+      if (out_p) out_p[k] = p;
+      if (out_quot) out_quot[k] = quot;
+      if (out_rem) out_rem[k] = rem;
+    }
+  }
+  stats->checksum = checksum;
+  if (bad)
+    return clw_fail(CLW_ERR_CHECK, "Wilson quotient check failed (rem != 0) in [%llu, %llu)",
+                    (unsigned long long)S->pmin, (unsigned long long)S->pmax);
+  return CLW_OK;
+}
+
+extern "C" int clw_wilson_range(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
+                                clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot, uint64_t* out_rem,
+                                uint64_t out_cap) {
+  if (!ctx || !stats) return clw_fail(CLW_ERR_ARG, "clw_wilson_range: null argument");
+  memset(stats, 0, sizeof(*stats));
+  clw_session* S = nullptr;
+  CK(clw_session_create(ctx, pmin, pmax, opt, &S));
+  int rc = clw_session_run(S, 2, S->maxT + 1);
+  if (rc == CLW_OK) rc = clw_session_iterate(S);
+  if (rc == CLW_OK) rc = clw_session_finalize(S, stats, out_p, out_quot, out_rem, out_cap);
+  clw_session_destroy(S);
+  return rc;
+}

@@ -1,0 +1,217 @@
+// clw_arith.h -- exact integer arithmetic mod p^2 for gfx950 (device side).
+//
+// Residues mod N = p^2 are held as L limbs of 30 bits (little-endian), value
+// sum v[i] * 2^(30 i). Multiplication is Montgomery's REDC with R = 2^(30 L),
+// computed column by column ("finely integrated product scanning"): every
+// 30x30-bit partial product is one v_mad_u64_u32 into a 64-bit column
+// accumulator. 30-bit limbs leave 4 bits of headroom, so a column of up to
+// 2L <= 16 partial products never overflows and no carry chains are needed.
+//
+// Bounds used throughout (N odd, 2N < R):
+//   mul_plain(a, q): a < 2N, q < R/2          ->  a q R^-1 mod N, result < 2N
+//   mul_full(a, b) : a, b < 2N                ->  a b R^-1 mod N, result < 2N
+//                    (one conditional subtraction; 4N^2/R + N < 3N)
+//
+// The reference keeps residues as two base-p digits with a digit-wise
+// Montgomery radix 2^64 (kernels/common.cl:115-260). The value computed for a
+// test prime is the same integer mod p^2; only the representation differs.
+#pragma once
+#include "clw_hd.h"
+
+namespace clw {
+
+constexpr uint32_t LB = 30;
+constexpr uint32_t LMASK = (1u << LB) - 1;
+
+typedef unsigned __int128 u128;
+
+template <int L>
+struct Res {
+  uint32_t v[L];
+};
+
+CLW_HD uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) {
+  return (uint64_t)a * b + c;
+}
+
+template <int L>
+CLW_HD Res<L> res_from_u128(u128 x) {
+  Res<L> r;
+#pragma unroll
+  for (int i = 0; i < L; ++i) r.v[i] = (uint32_t)(x >> (LB * i)) & LMASK;
+  return r;
+}
+
+template <int L>
+CLW_HD u128 res_to_u128(const Res<L>& a) {
+  u128 x = 0;
+#pragma unroll
+  for (int i = L - 1; i >= 0; --i) x = (x << LB) | a.v[i];
+  return x;
+}
+
+template <int L>
+CLW_HD Res<L> res_small(uint32_t s) {
+  Res<L> r;
+#pragma unroll
+  for (int i = 0; i < L; ++i) r.v[i] = 0;
+  r.v[0] = s;
+  return r;
+}
+
+// Montgomery product a*b*R^-1 mod n (lazy; see the bounds in the header).
+template <int L>
+CLW_HD Res<L> montmul(const Res<L>& a, const Res<L>& b, const Res<L>& n, uint32_t ninv) {
+  uint32_t m[L];
+  Res<L> r;
+  uint64_t c = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * L - 1; ++k) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int j = k - i;
+      if (j >= 0 && j < L) c = mad32(a.v[i], b.v[j], c);
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int j = k - i;
+      if (i < k && j >= 0 && j < L) c = mad32(m[i], n.v[j], c);
+    }
+    if (k < L) {
+      m[k] = ((uint32_t)c * ninv) & LMASK;
+      c = mad32(m[k], n.v[0], c);
+      c >>= LB;
+    } else {
+      r.v[k - L] = (uint32_t)c & LMASK;
+      c >>= LB;
+    }
+  }
+  r.v[L - 1] = (uint32_t)c;
+  return r;
+}
+
+// r - n if r >= n, else r (limb-wise borrow).
+template <int L>
+CLW_HD Res<L> cond_sub(const Res<L>& r, const Res<L>& n) {
+  Res<L> t;
+  int32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    int32_t d = (int32_t)r.v[i] - (int32_t)n.v[i] - borrow;
+    borrow = d < 0 ? 1 : 0;
+    t.v[i] = (uint32_t)d & LMASK;
+  }
+  return borrow ? r : t;
+}
+
+template <int L>
+CLW_HD Res<L> mul_plain(const Res<L>& a, const Res<L>& q, const Res<L>& n, uint32_t ninv) {
+  return montmul<L>(a, q, n, ninv);
+}
+
+template <int L>
+CLW_HD Res<L> mul_full(const Res<L>& a, const Res<L>& b, const Res<L>& n, uint32_t ninv) {
+  return cond_sub<L>(montmul<L>(a, b, n, ninv), n);
+}
+
+// a + b mod n for canonical a, b < n.
+template <int L>
+CLW_HD Res<L> add_mod(const Res<L>& a, const Res<L>& b, const Res<L>& n) {
+  Res<L> s;
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    uint32_t t = a.v[i] + b.v[i] + carry;
+    s.v[i] = t & LMASK;
+    carry = t >> LB;
+  }
+  return cond_sub<L>(s, n);
+}
+
+// a - b mod n for canonical a, b < n.
+template <int L>
+CLW_HD Res<L> sub_mod(const Res<L>& a, const Res<L>& b, const Res<L>& n) {
+  Res<L> t;
+  int32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    int32_t d = (int32_t)a.v[i] - (int32_t)b.v[i] - borrow;
+    borrow = d < 0 ? 1 : 0;
+    t.v[i] = (uint32_t)d & LMASK;
+  }
+  if (borrow) {
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      uint32_t s = t.v[i] + n.v[i] + carry;
+      t.v[i] = s & LMASK;
+      carry = s >> LB;
+    }
+  }
+  return t;
+}
+
+// Per-test-prime Montgomery context.
+template <int L>
+struct Ctx {
+  Res<L> n;       // N = p^2
+  uint32_t ninv;  // -N^-1 mod 2^30
+  Res<L> one;     // R mod N      (Montgomery form of 1)
+  Res<L> r2;      // R^2 mod N    (Montgomery form of R)
+};
+
+// -n0^-1 mod 2^30 for odd n0 (Newton iteration on 32-bit words).
+CLW_HD uint32_t neg_inv30(uint32_t n0) {
+  uint32_t inv = n0;  // correct to 3 bits for odd n0
+#pragma unroll
+  for (int i = 0; i < 4; ++i) inv *= 2u - n0 * inv;
+  return (0u - inv) & LMASK;
+}
+
+// 2^k mod m for m < 2^126 by doubling (used once per test prime).
+CLW_HD u128 pow2_mod(uint32_t k, u128 m) {
+  u128 x = 1 % m;
+  for (uint32_t i = 0; i < k; ++i) {
+    x <<= 1;
+    if (x >= m) x -= m;
+  }
+  return x;
+}
+
+// Montgomery form of R^e: returns R^(e+1) mod N (lazy < 2N). e is any 64-bit
+// exponent; the loop branches on e's bits (uniform when e is uniform).
+template <int L>
+CLW_HD Res<L> pow_r(const Ctx<L>& c, uint64_t e) {
+  if (e == 0) return c.one;
+  Res<L> r = c.r2;
+  const int top = 63 - __builtin_clzll(e);
+  for (int b = top - 1; b >= 0; --b) {
+    r = mul_full<L>(r, r, c.n, c.ninv);
+    if ((e >> b) & 1) r = mul_full<L>(r, c.r2, c.n, c.ninv);
+  }
+  return r;
+}
+
+// Montgomery power of a value already in Montgomery form (x R): (x^e) R.
+template <int L>
+CLW_HD Res<L> mont_pow(const Ctx<L>& c, const Res<L>& xm, uint64_t e) {
+  if (e == 0) return c.one;
+  Res<L> r = xm;
+  const int top = 63 - __builtin_clzll(e);
+  for (int b = top - 1; b >= 0; --b) {
+    r = mul_full<L>(r, r, c.n, c.ninv);
+    if ((e >> b) & 1) r = mul_full<L>(r, xm, c.n, c.ninv);
+  }
+  return r;
+}
+
+// Drift bookkeeping: a value "a with drift d" stands for a * R^d mod N.
+// Returns the canonical value (< N) of a * R^d.
+template <int L>
+CLW_HD Res<L> undrift(const Ctx<L>& c, const Res<L>& a, uint64_t d) {
+  const Res<L> w = pow_r<L>(c, d);  // R^(d+1)
+  Res<L> r = mul_full<L>(a, w, c.n, c.ninv);
+  return cond_sub<L>(r, c.n);
+}
+
+}  // namespace clw

@@ -1,0 +1,94 @@
+// clw_hostcheck.cpp -- the device arithmetic headers compiled for the host
+// CPU, exported for unit tests (tests/test_arith_host.py). This library is a
+// test harness for clw_arith.h / clw_nt64.h; the product runs them on the GPU.
+#include <string.h>
+#include "clw_arith.h"
+#include "clw_nt64.h"
+
+using namespace clw;
+
+template <int L>
+static void montmul_t(const uint64_t a[2], const uint64_t b[2], const uint64_t n[2], uint64_t out[2]) {
+  const u128 A = ((u128)a[1] << 64) | a[0], B = ((u128)b[1] << 64) | b[0], Nv = ((u128)n[1] << 64) | n[0];
+  const Res<L> ra = res_from_u128<L>(A), rb = res_from_u128<L>(B), rn = res_from_u128<L>(Nv);
+  const uint32_t ninv = neg_inv30(rn.v[0]);
+  const Res<L> r = montmul<L>(ra, rb, rn, ninv);
+  const u128 R = res_to_u128<L>(r);
+  out[0] = (uint64_t)R;
+  out[1] = (uint64_t)(R >> 64);
+}
+
+template <int L>
+static Ctx<L> make_ctx(uint64_t p) {
+  const u128 N = (u128)p * p;
+  Ctx<L> c;
+  c.n = res_from_u128<L>(N);
+  c.ninv = neg_inv30(c.n.v[0]);
+  const u128 one = pow2_mod(30 * L, N);
+  u128 r2 = one;
+  for (int i = 0; i < 30 * L; ++i) { r2 <<= 1; if (r2 >= N) r2 -= N; }
+  c.one = res_from_u128<L>(one);
+  c.r2 = res_from_u128<L>(r2);
+  return c;
+}
+
+template <int L>
+static void undrift_t(uint64_t p, const uint64_t a[2], uint64_t d, uint64_t out[2]) {
+  const Ctx<L> c = make_ctx<L>(p);
+  const u128 A = ((u128)a[1] << 64) | a[0];
+  const Res<L> r = undrift<L>(c, res_from_u128<L>(A), d);
+  const u128 R = res_to_u128<L>(r);
+  out[0] = (uint64_t)R;
+  out[1] = (uint64_t)(R >> 64);
+}
+
+extern "C" {
+
+int clwh_montmul(int L, const uint64_t* a, const uint64_t* b, const uint64_t* n, uint64_t* out) {
+  switch (L) {
+    case 2: montmul_t<2>(a, b, n, out); return 0;
+    case 3: montmul_t<3>(a, b, n, out); return 0;
+    case 4: montmul_t<4>(a, b, n, out); return 0;
+    case 5: montmul_t<5>(a, b, n, out); return 0;
+  }
+  return -1;
+}
+
+int clwh_undrift(int L, uint64_t p, const uint64_t* a, uint64_t d, uint64_t* out) {
+  switch (L) {
+    case 2: undrift_t<2>(p, a, d, out); return 0;
+    case 3: undrift_t<3>(p, a, d, out); return 0;
+    case 4: undrift_t<4>(p, a, d, out); return 0;
+    case 5: undrift_t<5>(p, a, d, out); return 0;
+  }
+  return -1;
+}
+
+uint32_t clwh_neg_inv30(uint32_t n0) { return neg_inv30(n0); }
+uint64_t clwh_inv64(uint64_t n) { return inv64(n); }
+uint64_t clwh_legendre(uint64_t q, uint64_t T) { return legendre_exp(q, T); }
+uint64_t clwh_isqrt(uint64_t n) { return isqrt64(n); }
+uint64_t clwh_inv_mod(uint64_t a, uint64_t n) { return inv_mod64(a, n); }
+int clwh_mr_bases_needed(uint64_t n) { return mr_bases_needed(n); }
+
+// Full deterministic primality test as the sieve kernel applies it to a
+// candidate n > 37 that survived trial division (base 2, then extra bases).
+int clwh_is_prime_mr(uint64_t n, int nbases) {
+  if (n < 2) return 0;
+  if ((n & 1) == 0) return n == 2;
+  const uint32_t small[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (int i = 0; i < 12; ++i) {
+    if (n == small[i]) return 1;
+    if (n % small[i] == 0) return 0;
+  }
+  const M64 m = m64_make(n);
+  const int s = __builtin_ctzll(n - 1);
+  const uint64_t d = (n - 1) >> s;
+  if (!sprp2(m, d, s)) return 0;
+  return mr_extra(m, d, s, nbases) ? 1 : 0;
+}
+
+int64_t clwh_two_squares_a(uint64_t p) { return two_squares_a(p); }
+void clwh_eisenstein_cu(uint64_t p, int64_t* c, int64_t* u) { eisenstein_cu(p, c, u); }
+
+}  // extern "C"

@@ -1,0 +1,778 @@
+// clw_kernels.hip -- gfx950 kernels of the Wilson-prime hot path and their
+// typed C-ABI launchers (include/clwilson.h).
+//
+// Data flow per range (cl_wilson.cpp:1442-1974 restated for MI355X):
+//   k_setup        per test prime: N = p^2, -N^-1 mod 2^30, R mod N, R^2 mod N
+//   k_sieve_pack   per segment: exact sieving primes, paired into 32-byte
+//                  records with their Legendre exponents per type
+//   k_mul_records  per (64-prime tile, chunk of blocks): running product of
+//                  the records' prime powers, one test prime per lane, the
+//                  records read as wave-uniform scalar loads
+//   k_mul_side     the same for prime powers kept out of the records
+//   k_reduce       folds the chunk products into the running residues
+//   k_iterate      T_p! = T_t! * prod (T_t, T_p], one wave per test prime
+//   k_finalize     (p-1)! + 1 mod p^2 from the type identities -> quot, rem
+#include <hip/hip_runtime.h>
+
+#include "clw_arith.h"
+#include "clw_internal.h"
+#include "clw_nt64.h"
+
+using namespace clw;
+
+// ---------------------------------------------------------------------------
+// Sieve constants: odd primes 7..1021 and, for each prime q and each of the
+// 8 residues r coprime to 30, the residue k mod 30 of multipliers with
+// q*k = r (mod 30). Filled once per process by clw_upload_sieve_tables().
+// ---------------------------------------------------------------------------
+__constant__ uint32_t c_sp[CLW_SP_COUNT];
+__constant__ uint8_t c_kres[CLW_SP_COUNT * 8];
+
+static const uint32_t kWheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+__constant__ uint32_t c_wheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+
+constexpr uint32_t TURNS = CLW_BLOCK_SPAN / 30;  // 2048 wheel turns per block
+constexpr uint32_t CAND_CAP = 8 * TURNS;         // every wheel position
+constexpr uint32_t PRIME_CAP = 2 * CLW_REC_CAP;
+constexpr uint64_t TRIAL_SQ = 1021ull * 1021ull;  // below this a survivor is prime
+constexpr uint32_t E_SIDE = 1u << 20;             // record exponents stay below
+
+int clw_upload_sieve_tables() {
+  uint32_t sp[CLW_SP_COUNT];
+  uint8_t kres[CLW_SP_COUNT * 8];
+  int n = 0;
+  for (uint32_t q = 7; q <= 1021 && n < (int)CLW_SP_COUNT; q += 2) {
+    bool pr = true;
+    for (uint32_t d = 3; d * d <= q; d += 2)
+      if (q % d == 0) { pr = false; break; }
+    if (!pr) continue;
+    sp[n] = q;
+    uint32_t qinv = 0;
+    for (uint32_t k = 1; k < 30; ++k)
+      if ((q * k) % 30 == 1) { qinv = k; break; }
+    for (int r = 0; r < 8; ++r) kres[n * 8 + r] = (uint8_t)((kWheel[r] * qinv) % 30);
+    ++n;
+  }
+  if (n != (int)CLW_SP_COUNT) return clw_fail(CLW_ERR_STATE, "sieve table size %d", n);
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sp), sp, sizeof(sp));
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_kres), kres, sizeof(kres));
+  if (e != hipSuccess) return clw_fail_hip(e, "upload sieve tables");
+  return CLW_OK;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads).
+__device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t s = wsum[w];
+    if ((uint32_t)w < wave) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+__device__ __forceinline__ void side_push(uint64_t* __restrict__ side, uint32_t* side_count, uint32_t t,
+                                          uint32_t cap, uint32_t* flags, uint64_t q, uint64_t e) {
+  const uint32_t idx = atomicAdd(&side_count[t], 1u);
+  if (idx < cap) {
+    side[2 * (size_t)idx] = q;
+    side[2 * (size_t)idx + 1] = e;
+  } else {
+    atomicOr(flags, CLW_FLAG_SIDE_OVERFLOW);
+  }
+}
+
+// One block = CLW_BLOCK_SPAN consecutive integers from base + b*SPAN.
+__global__ __launch_bounds__(256) void k_sieve_pack(uint64_t base, uint64_t lo, uint64_t hi, uint64_t T0,
+                                                    uint64_t T1, uint64_t T2, uint32_t nbases, uint32_t side_cap,
+                                                    uint32_t* __restrict__ rec, uint32_t* __restrict__ nrec,
+                                                    uint32_t* __restrict__ nprimes, uint64_t* __restrict__ side0,
+                                                    uint64_t* __restrict__ side1, uint64_t* __restrict__ side2,
+                                                    uint32_t* __restrict__ side_count, uint32_t* __restrict__ flags,
+                                                    unsigned long long* __restrict__ total) {
+  __shared__ uint32_t bm[TURNS / 4];      // one byte per wheel turn, bit r <-> residue c_wheel[r]
+  __shared__ uint16_t cand[CAND_CAP];     // offsets of survivors, increasing
+  __shared__ uint16_t prm[PRIME_CAP];     // offsets of primes, increasing
+  __shared__ uint32_t pass[CAND_CAP / 32];
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t s_fail;
+
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  const uint64_t blo = base + (uint64_t)b * CLW_BLOCK_SPAN;
+  const uint64_t bhi = blo + CLW_BLOCK_SPAN;
+  const uint64_t vlo = blo > lo ? blo : lo;
+  const uint64_t vhi = bhi < hi ? bhi : hi;
+
+  for (uint32_t i = tid; i < TURNS / 4; i += 256) bm[i] = 0xFFFFFFFFu;
+  for (uint32_t i = tid; i < CAND_CAP / 32; i += 256) pass[i] = 0;
+  if (tid == 0) s_fail = 0;
+  __syncthreads();
+
+  // Trial division by 7..1021 on the wheel: strike multiples q*k, k >= q.
+  for (uint32_t pr = tid; pr < CLW_SP_COUNT * 8; pr += 256) {
+    const uint32_t qi = pr >> 3, ri = pr & 7;
+    const uint64_t q = c_sp[qi];
+    const uint64_t start = blo > q * q ? blo : q * q;
+    if (start >= bhi) continue;
+    const uint64_t k0 = (start + q - 1) / q;
+    const uint32_t km = (uint32_t)(k0 % 30), kr = c_kres[pr];
+    const uint64_t n = q * (k0 + (kr + 30 - km) % 30);
+    if (n >= bhi) continue;
+    for (uint32_t turn = (uint32_t)((n - blo) / 30); turn < TURNS; turn += (uint32_t)q)
+      atomicAnd(&bm[turn >> 2], ~(1u << ((turn & 3) * 8 + ri)));
+  }
+  __syncthreads();
+
+  // Survivors in [vlo, vhi), thread tid owns turns [8 tid, 8 tid + 8).
+  const bool interior = (blo >= lo) && (bhi <= hi);
+  uint64_t bits = ((uint64_t)bm[2 * tid + 1] << 32) | bm[2 * tid];
+  if (!interior) {
+    uint64_t keep = 0;
+    for (int j = 0; j < 64; ++j) {
+      const uint64_t nn = blo + 30ull * (8 * tid + (j >> 3)) + c_wheel[j & 7];
+      if (nn >= vlo && nn < vhi) keep |= 1ull << j;
+    }
+    bits &= keep;
+  }
+  uint32_t ncand;
+  uint32_t pos = block_exscan((uint32_t)__popcll(bits), wsum, &ncand);
+  while (bits) {
+    const int j = __ffsll((unsigned long long)bits) - 1;
+    bits &= bits - 1;
+    cand[pos++] = (uint16_t)(30u * (8 * tid + (j >> 3)) + c_wheel[j & 7]);
+  }
+  __syncthreads();
+
+  // Base-2 strong test on every survivor >= 1021^2.
+  for (uint32_t j = tid; j < ncand; j += 256) {
+    const uint64_t nn = blo + cand[j];
+    bool ok = true;
+    if (nn >= TRIAL_SQ) {
+      const M64 m = m64_make(nn);
+      const int s = __builtin_ctzll(nn - 1);
+      ok = sprp2(m, (nn - 1) >> s, s);
+    }
+    if (ok) atomicOr(&pass[j >> 5], 1u << (j & 31));
+  }
+  __syncthreads();
+
+  // Compact the passes (in order) into prm[].
+  uint32_t np;
+  {
+    const uint32_t per = (ncand + 255) / 256;  // candidates per thread, contiguous
+    const uint32_t j0 = tid * per, j1 = (j0 + per < ncand) ? j0 + per : ncand;
+    uint32_t c = 0;
+    for (uint32_t j = j0; j < j1; ++j) c += (pass[j >> 5] >> (j & 31)) & 1;
+    uint32_t o = block_exscan(c, wsum, &np);
+    for (uint32_t j = j0; j < j1; ++j)
+      if ((pass[j >> 5] >> (j & 31)) & 1) {
+        if (o < PRIME_CAP) prm[o] = cand[j];
+        ++o;
+      }
+  }
+  if (np > PRIME_CAP) {
+    if (tid == 0) atomicOr(flags, CLW_FLAG_SIEVE_OVERFLOW);
+    np = PRIME_CAP;
+  }
+  __syncthreads();
+
+  // Remaining Miller-Rabin bases on the (dense) base-2 passes.
+  if (nbases > 1) {
+    for (uint32_t j = tid; j < np; j += 256) {
+      const uint64_t nn = blo + prm[j];
+      if (nn < TRIAL_SQ) continue;
+      const M64 m = m64_make(nn);
+      const int s = __builtin_ctzll(nn - 1);
+      if (!mr_extra(m, (nn - 1) >> s, s, (int)nbases)) {
+        prm[j] = 0;  // offset 0 is never a prime (blo is a multiple of 30)
+        atomicOr(&s_fail, 1u);
+      }
+    }
+    __syncthreads();
+    if (s_fail) {  // rare: drop base-2 pseudoprimes, keeping the order
+      if (tid == 0) {
+        uint32_t o = 0;
+        for (uint32_t j = 0; j < np; ++j)
+          if (prm[j]) prm[o++] = prm[j];
+        s_fail = o;
+      }
+      __syncthreads();
+      np = s_fail;
+    }
+  }
+
+  // Pair consecutive primes into records {Q limbs[5], e0, e1, e2}.
+  uint32_t nr = (np + 1) / 2;
+  if (nr > CLW_REC_CAP) {
+    if (tid == 0) atomicOr(flags, CLW_FLAG_REC_OVERFLOW);
+    nr = CLW_REC_CAP;
+  }
+  const uint64_t T[3] = {T0, T1, T2};
+  uint64_t* const sides[3] = {side0, side1, side2};
+  uint32_t* const rb = rec + (size_t)b * CLW_REC_CAP * CLW_REC_WORDS;
+  for (uint32_t m = tid; m < nr; m += 256) {
+    const uint64_t qa = blo + prm[2 * m];
+    const bool single = (2 * m + 1 >= np);
+    const uint64_t qb = single ? 0 : blo + prm[2 * m + 1];
+    const u128 Q = single ? (u128)qa : (u128)qa * qb;
+    uint32_t w[8];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) w[l] = (uint32_t)(Q >> (30 * l)) & LMASK;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      uint32_t me = 0;
+      if (T[t] != 0 && qa <= T[t]) {
+        const uint64_t ea = legendre_exp(qa, T[t]);
+        const uint64_t eb = single ? ea : legendre_exp(qb, T[t]);
+        const uint64_t mm = eb < E_SIDE ? eb : 0;  // ea >= eb: q ascending
+        me = (uint32_t)mm;
+        if (ea > mm) side_push(sides[t], side_count, t, side_cap, flags, qa, ea - mm);
+        if (!single && eb > mm) side_push(sides[t], side_count, t, side_cap, flags, qb, eb - mm);
+      }
+      w[5 + t] = me;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(rb + (size_t)m * CLW_REC_WORDS);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+  if (tid == 0) {
+    nrec[b] = nr;
+    nprimes[b] = np;
+    atomicAdd(total, (unsigned long long)np);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-test-prime constants and residues
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_setup(const uint64_t* __restrict__ P, const uint32_t* __restrict__ type,
+                                               uint64_t* __restrict__ tp_target, uint64_t* __restrict__ n_lo,
+                                               uint64_t* __restrict__ n_hi, uint64_t* __restrict__ one_lo,
+                                               uint64_t* __restrict__ one_hi, uint64_t* __restrict__ r2_lo,
+                                               uint64_t* __restrict__ r2_hi, uint32_t* __restrict__ ninv,
+                                               uint64_t* __restrict__ acc_lo, uint64_t* __restrict__ acc_hi,
+                                               uint32_t count, int resume) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t p = P[i];
+  const uint32_t t = type[i];
+  const u128 N = (u128)p * p;
+  const u128 one = pow2_mod(30 * L, N);
+  u128 r2 = one;
+  for (int k = 0; k < 30 * L; ++k) {
+    r2 <<= 1;
+    if (r2 >= N) r2 -= N;
+  }
+  tp_target[i] = (t == 0) ? (p - 1) / 6 : (t == 1) ? (p - 1) / 4 : (p - 1) / 2;
+  n_lo[i] = (uint64_t)N;
+  n_hi[i] = (uint64_t)(N >> 64);
+  one_lo[i] = (uint64_t)one;
+  one_hi[i] = (uint64_t)(one >> 64);
+  r2_lo[i] = (uint64_t)r2;
+  r2_hi[i] = (uint64_t)(r2 >> 64);
+  ninv[i] = neg_inv30((uint32_t)N & LMASK);
+  if (!resume) {
+    acc_lo[i] = 1;
+    acc_hi[i] = 0;
+  }
+}
+
+template <int L>
+__device__ __forceinline__ Ctx<L> load_ctx(const uint64_t* __restrict__ n_lo, const uint64_t* __restrict__ n_hi,
+                                          const uint64_t* __restrict__ one_lo, const uint64_t* __restrict__ one_hi,
+                                          const uint64_t* __restrict__ r2_lo, const uint64_t* __restrict__ r2_hi,
+                                          const uint32_t* __restrict__ ninv, uint32_t i) {
+  Ctx<L> c;
+  c.n = res_from_u128<L>(((u128)n_hi[i] << 64) | n_lo[i]);
+  c.one = res_from_u128<L>(((u128)one_hi[i] << 64) | one_lo[i]);
+  c.r2 = res_from_u128<L>(((u128)r2_hi[i] << 64) | r2_lo[i]);
+  c.ninv = ninv[i];
+  return c;
+}
+
+#define CTX_PARAMS                                                                                         \
+  const uint64_t *__restrict__ n_lo, const uint64_t *__restrict__ n_hi, const uint64_t *__restrict__ one_lo, \
+      const uint64_t *__restrict__ one_hi, const uint64_t *__restrict__ r2_lo,                               \
+      const uint64_t *__restrict__ r2_hi, const uint32_t *__restrict__ ninv
+#define CTX_ARGS(t) t->n_lo, t->n_hi, t->one_lo, t->one_hi, t->r2_lo, t->r2_hi, t->ninv
+#define LOAD_CTX(i) load_ctx<L>(n_lo, n_hi, one_lo, one_hi, r2_lo, r2_hi, ninv, (i))
+
+// A *= X^e, with drift bookkeeping (a value with drift d stands for v R^d).
+template <int L>
+__device__ __forceinline__ void flush_run(const Ctx<L>& c, Res<L>& A, uint64_t& dA, const Res<L>& X, uint64_t dX,
+                                          uint32_t e) {
+  Res<L> Y = X;
+  uint64_t dY = dX;
+  if (e > 1) {
+    const int top = 31 - __builtin_clz(e);
+    for (int bit = top - 1; bit >= 0; --bit) {
+      Y = mul_full<L>(Y, Y, c.n, c.ninv);
+      dY = 2 * dY + 1;
+      if ((e >> bit) & 1) {
+        Y = mul_full<L>(Y, X, c.n, c.ninv);
+        dY += dX + 1;
+      }
+    }
+  }
+  A = mul_full<L>(A, Y, c.n, c.ninv);
+  dA += dY + 1;
+}
+
+// The hot kernel. grid = (ceil(tp_n/256), K); lane = one test prime; chunk y
+// covers sieve blocks [nblk*y/K, nblk*(y+1)/K). Records are read with
+// wave-uniform addresses (scalar loads); consecutive records with the same
+// exponent form a run X = prod Q, applied once as A *= X^e.
+template <int L>
+__global__ __launch_bounds__(256) void k_mul_records(CTX_PARAMS, const uint32_t* __restrict__ rec,
+                                                     const uint32_t* __restrict__ nrec, uint64_t* __restrict__ part_lo,
+                                                     uint64_t* __restrict__ part_hi, uint32_t tp_off, uint32_t tp_n,
+                                                     uint32_t nblk, uint32_t K, uint32_t part_stride, uint32_t type) {
+  const uint32_t li = blockIdx.x * 256 + threadIdx.x;
+  const bool live = li < tp_n;
+  const Ctx<L> c = LOAD_CTX(tp_off + (live ? li : 0));
+  const uint32_t chunk = blockIdx.y;
+  const uint32_t b0 = (uint32_t)((uint64_t)nblk * chunk / K);
+  const uint32_t b1 = (uint32_t)((uint64_t)nblk * (chunk + 1) / K);
+  const uint32_t ew = 5 + type;
+
+  Res<L> A = res_small<L>(1), X = res_small<L>(1);
+  uint64_t dA = 0, dX = 0;
+  uint32_t cur = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t nr = nrec[b];
+    const uint32_t* __restrict__ rp = rec + (size_t)b * CLW_REC_CAP * CLW_REC_WORDS;
+    for (uint32_t r = 0; r < nr; ++r) {
+      const uint32_t* __restrict__ w = rp + (size_t)r * CLW_REC_WORDS;
+      const uint32_t e = w[ew];
+      if (e == 0) continue;
+      if (e != cur) {
+        if (cur) flush_run<L>(c, A, dA, X, dX, cur);
+        X = res_small<L>(1);
+        dX = 0;
+        cur = e;
+      }
+      Res<L> q;
+#pragma unroll
+      for (int l = 0; l < L; ++l) q.v[l] = w[l];
+      X = mul_plain<L>(X, q, c.n, c.ninv);
+      ++dX;
+    }
+  }
+  if (cur) flush_run<L>(c, A, dA, X, dX, cur);
+  const Res<L> P = undrift<L>(c, A, dA);
+  if (live) {
+    const u128 v = res_to_u128<L>(P);
+    const size_t o = (size_t)chunk * part_stride + li;
+    part_lo[o] = (uint64_t)v;
+    part_hi[o] = (uint64_t)(v >> 64);
+  }
+}
+
+// Variant for ranges with few test primes: a wave owns one test prime and its
+// 64 lanes stride over the records of each block (lane l takes records
+// l, l+64, ...; each lane's subsequence is still sorted, so runs work per
+// lane). The lane products are combined by a butterfly with drift tracking.
+// grid = (ceil(tp_n/4), K).
+template <int L>
+__global__ __launch_bounds__(256) void k_mul_records_lanes(CTX_PARAMS, const uint32_t* __restrict__ rec,
+                                                           const uint32_t* __restrict__ nrec,
+                                                           uint64_t* __restrict__ part_lo,
+                                                           uint64_t* __restrict__ part_hi, uint32_t tp_off,
+                                                           uint32_t tp_n, uint32_t nblk, uint32_t K,
+                                                           uint32_t part_stride, uint32_t type) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wi >= tp_n) return;  // whole wave
+  const Ctx<L> c = LOAD_CTX(tp_off + wi);
+  const uint32_t chunk = blockIdx.y;
+  const uint32_t b0 = (uint32_t)((uint64_t)nblk * chunk / K);
+  const uint32_t b1 = (uint32_t)((uint64_t)nblk * (chunk + 1) / K);
+  const uint32_t ew = 5 + type;
+
+  Res<L> A = res_small<L>(1), X = res_small<L>(1);
+  uint64_t dA = 0, dX = 0;
+  uint32_t cur = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t nr = nrec[b];
+    const uint32_t* __restrict__ rp = rec + (size_t)b * CLW_REC_CAP * CLW_REC_WORDS;
+    for (uint32_t r = lane; r < nr; r += 64) {
+      const uint4* w4 = reinterpret_cast<const uint4*>(rp + (size_t)r * CLW_REC_WORDS);
+      const uint4 lo4 = w4[0], hi4 = w4[1];
+      const uint32_t w[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+      const uint32_t e = w[ew];
+      if (e == 0) continue;
+      if (e != cur) {
+        if (cur) flush_run<L>(c, A, dA, X, dX, cur);
+        X = res_small<L>(1);
+        dX = 0;
+        cur = e;
+      }
+      Res<L> q;
+#pragma unroll
+      for (int l = 0; l < L; ++l) q.v[l] = w[l];
+      X = mul_plain<L>(X, q, c.n, c.ninv);
+      ++dX;
+    }
+  }
+  if (cur) flush_run<L>(c, A, dA, X, dX, cur);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    Res<L> Y;
+#pragma unroll
+    for (int l = 0; l < L; ++l) Y.v[l] = __shfl_xor(A.v[l], off, 64);
+    const uint32_t dlo = __shfl_xor((uint32_t)dA, off, 64), dhi = __shfl_xor((uint32_t)(dA >> 32), off, 64);
+    A = mul_full<L>(A, Y, c.n, c.ninv);
+    dA = dA + (((uint64_t)dhi << 32) | dlo) + 1;
+  }
+  if (lane == 0) {
+    const u128 v = res_to_u128<L>(undrift<L>(c, A, dA));
+    const size_t o = (size_t)chunk * part_stride + wi;
+    part_lo[o] = (uint64_t)v;
+    part_hi[o] = (uint64_t)(v >> 64);
+  }
+}
+
+// Side list: {q, e} pairs with arbitrary 64-bit exponents (Montgomery powers).
+template <int L>
+__global__ __launch_bounds__(256) void k_mul_side(CTX_PARAMS, const uint64_t* __restrict__ side,
+                                                  const uint32_t* __restrict__ side_count, uint32_t side_cap,
+                                                  uint64_t* __restrict__ part_lo, uint64_t* __restrict__ part_hi,
+                                                  uint32_t tp_off, uint32_t tp_n, uint32_t K, uint32_t part_stride,
+                                                  uint32_t type) {
+  const uint32_t li = blockIdx.x * 256 + threadIdx.x;
+  const bool live = li < tp_n;
+  const Ctx<L> c = LOAD_CTX(tp_off + (live ? li : 0));
+  uint32_t cnt = side_count[type];
+  if (cnt > side_cap) cnt = side_cap;
+  const uint32_t chunk = blockIdx.y;
+  const uint32_t s0 = (uint32_t)((uint64_t)cnt * chunk / K), s1 = (uint32_t)((uint64_t)cnt * (chunk + 1) / K);
+  Res<L> A = c.one;
+  for (uint32_t s = s0; s < s1; ++s) {
+    const uint64_t q = side[2 * (size_t)s], e = side[2 * (size_t)s + 1];
+    const Res<L> qm = mul_plain<L>(c.r2, res_from_u128<L>(q), c.n, c.ninv);  // q R
+    A = mul_full<L>(A, mont_pow<L>(c, qm, e), c.n, c.ninv);
+  }
+  const Res<L> P = cond_sub<L>(mul_plain<L>(A, res_small<L>(1), c.n, c.ninv), c.n);
+  if (live) {
+    const u128 v = res_to_u128<L>(P);
+    const size_t o = (size_t)chunk * part_stride + li;
+    part_lo[o] = (uint64_t)v;
+    part_hi[o] = (uint64_t)(v >> 64);
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_reduce(CTX_PARAMS, uint64_t* __restrict__ acc_lo,
+                                                uint64_t* __restrict__ acc_hi, const uint64_t* __restrict__ part_lo,
+                                                const uint64_t* __restrict__ part_hi, uint32_t tp_off, uint32_t tp_n,
+                                                uint32_t nparts, uint32_t part_stride) {
+  const uint32_t li = blockIdx.x * 256 + threadIdx.x;
+  if (li >= tp_n) return;
+  const uint32_t i = tp_off + li;
+  const Ctx<L> c = LOAD_CTX(i);
+  Res<L> A = res_from_u128<L>(((u128)acc_hi[i] << 64) | acc_lo[i]);
+  for (uint32_t k = 0; k < nparts; ++k) {
+    const size_t o = (size_t)k * part_stride + li;
+    A = mul_full<L>(A, res_from_u128<L>(((u128)part_hi[o] << 64) | part_lo[o]), c.n, c.ninv);
+  }
+  A = undrift<L>(c, A, nparts);
+  const u128 v = res_to_u128<L>(A);
+  acc_lo[i] = (uint64_t)v;
+  acc_hi[i] = (uint64_t)(v >> 64);
+}
+
+// One wave per test prime: lane l multiplies a contiguous even-length span of
+// (T_type, T_p], consecutive integers paired as n(n+1) (advanced by 4n+6).
+template <int L>
+__global__ __launch_bounds__(256) void k_iterate(CTX_PARAMS, const uint64_t* __restrict__ tp_target,
+                                                 const uint32_t* __restrict__ type, uint64_t* __restrict__ acc_lo,
+                                                 uint64_t* __restrict__ acc_hi, uint32_t count, uint64_t from0,
+                                                 uint64_t from1, uint64_t from2) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= count) return;  // whole wave
+  const Ctx<L> c = LOAD_CTX(i);
+  const uint32_t t = type[i];
+  const uint64_t from = t == 0 ? from0 : (t == 1 ? from1 : from2);
+  const uint64_t to = tp_target[i];
+  const uint64_t len = to > from ? to - from : 0;
+  uint64_t per = (len + 63) / 64;
+  per += per & 1;
+  const uint64_t s = from + 1 + (uint64_t)lane * per;
+  const uint64_t e = (s + per < to + 1) ? s + per : to + 1;
+  Res<L> X = res_small<L>(1);
+  uint64_t dX = 0;
+  if (s < e) {
+    uint64_t n = s;
+    u128 Q = (u128)n * (n + 1);
+    for (; n + 1 < e; n += 2) {
+      X = mul_plain<L>(X, res_from_u128<L>(Q), c.n, c.ninv);
+      ++dX;
+      Q += (u128)(4 * n + 6);
+    }
+    if (n < e) {
+      X = mul_plain<L>(X, res_from_u128<L>((u128)n), c.n, c.ninv);
+      ++dX;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    Res<L> Y;
+#pragma unroll
+    for (int l = 0; l < L; ++l) Y.v[l] = __shfl_xor(X.v[l], off, 64);
+    const uint32_t dlo = __shfl_xor((uint32_t)dX, off, 64), dhi = __shfl_xor((uint32_t)(dX >> 32), off, 64);
+    X = mul_full<L>(X, Y, c.n, c.ninv);
+    dX = dX + (((uint64_t)dhi << 32) | dlo) + 1;
+  }
+  if (lane == 0) {
+    Res<L> A = res_from_u128<L>(((u128)acc_hi[i] << 64) | acc_lo[i]);
+    A = mul_full<L>(A, X, c.n, c.ninv);
+    A = undrift<L>(c, A, dX + 1);
+    const u128 v = res_to_u128<L>(A);
+    acc_lo[i] = (uint64_t)v;
+    acc_hi[i] = (uint64_t)(v >> 64);
+  }
+}
+
+// x < d^2: quotient and remainder of x / d by shift-subtract on the low word.
+__device__ __forceinline__ void divmod_sq(u128 x, uint64_t d, uint64_t* q, uint64_t* r) {
+  u128 rem = x >> 64;  // < d
+  const uint64_t lo = (uint64_t)x;
+  uint64_t quo = 0;
+  for (int i = 63; i >= 0; --i) {
+    rem = (rem << 1) | ((lo >> i) & 1);
+    if (rem >= d) {
+      rem -= d;
+      quo |= 1ull << i;
+    }
+  }
+  *q = quo;
+  *r = (uint64_t)rem;
+}
+
+template <int L>
+struct Fin {
+  Ctx<L> c;
+  __device__ __forceinline__ Res<L> canon(const Res<L>& x) const { return cond_sub<L>(x, c.n); }
+  __device__ __forceinline__ Res<L> mm(const Res<L>& x, const Res<L>& y) const {
+    return canon(mul_full<L>(x, y, c.n, c.ninv));
+  }
+  // Montgomery form of a canonical value (< N).
+  __device__ __forceinline__ Res<L> to_m(u128 x) const {
+    return canon(mul_plain<L>(c.r2, res_from_u128<L>(x), c.n, c.ninv));
+  }
+  __device__ __forceinline__ Res<L> from_m(const Res<L>& y) const {
+    return canon(mul_plain<L>(y, res_small<L>(1), c.n, c.ninv));
+  }
+  __device__ __forceinline__ Res<L> neg(const Res<L>& y) const { return sub_mod<L>(res_small<L>(0), y, c.n); }
+  __device__ __forceinline__ Res<L> add(const Res<L>& x, const Res<L>& y) const { return add_mod<L>(x, y, c.n); }
+  __device__ __forceinline__ Res<L> sub(const Res<L>& x, const Res<L>& y) const { return sub_mod<L>(x, y, c.n); }
+  // Montgomery form of a signed 64-bit integer.
+  __device__ __forceinline__ Res<L> to_m_signed(int64_t v, u128 N) const {
+    const uint64_t a = v < 0 ? (uint64_t)0 - (uint64_t)v : (uint64_t)v;
+    const Res<L> m = to_m((u128)a % N);
+    return v < 0 ? neg(m) : m;
+  }
+  __device__ __forceinline__ Res<L> pow(const Res<L>& xm, uint64_t e) const { return canon(mont_pow<L>(c, xm, e)); }
+};
+
+template <int L>
+__global__ __launch_bounds__(256) void k_finalize(CTX_PARAMS, const uint64_t* __restrict__ P,
+                                                  const uint32_t* __restrict__ type,
+                                                  const uint64_t* __restrict__ acc_lo,
+                                                  const uint64_t* __restrict__ acc_hi, uint32_t count,
+                                                  uint64_t* __restrict__ quot, uint64_t* __restrict__ rem,
+                                                  int64_t* __restrict__ acu) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  Fin<L> f;
+  f.c = LOAD_CTX(i);
+  const uint64_t p = P[i];
+  const u128 N = (u128)p * p;
+  const uint32_t t = type[i];
+  const Res<L> one = f.canon(f.c.one);
+  const Res<L> V = f.to_m(((u128)acc_hi[i] << 64) | acc_lo[i]);  // T_p! in Montgomery form
+  const Res<L> two = f.add(one, one);
+  const Res<L> p2 = f.pow(two, p);  // 2^p
+  Res<L> r;
+  int64_t acu0 = 0, acu1 = 0;
+  if (t == 2) {
+    // ((p-1)/2)!^2 (1 - 2^p)   (cl_wilson.cpp:832-845)
+    r = f.mm(f.mm(V, V), f.sub(one, p2));
+  } else if (t == 1) {
+    // ((p-1)/4)!^4 (3 * 2^p - 4) (2a^2 - p)   (cl_wilson.cpp:800-831)
+    const int64_t a = two_squares_a(p);
+    acu0 = a;
+    const Res<L> v2 = f.mm(V, V);
+    const Res<L> w1 = f.sub(f.add(f.add(p2, p2), p2), f.add(two, two));
+    const uint64_t aa = (uint64_t)(a < 0 ? -a : a);
+    const u128 a2 = (u128)aa * aa;  // < p
+    const Res<L> w2 = f.sub(f.to_m(2 * a2), f.to_m((u128)p));
+    r = f.mm(f.mm(f.mm(v2, v2), w1), w2);
+  } else {
+    // ((p-1)/6)!^6 (-u^3 (2^p - 1) + 3pu) (p/c - c) (3^p - 1)/2   (cl_wilson.cpp:731-799)
+    int64_t cc, uu;
+    eisenstein_cu(p, &cc, &uu);
+    acu0 = uu;
+    acu1 = cc;
+    const Res<L> v2 = f.mm(V, V);
+    const Res<L> v6 = f.mm(f.mm(v2, v2), v2);
+    const Res<L> um = f.to_m_signed(uu, N);
+    const Res<L> pm = f.to_m((u128)p);
+    const Res<L> u3 = f.mm(f.mm(um, um), um);
+    const Res<L> pu = f.mm(pm, um);
+    const Res<L> w1 = f.add(f.neg(f.mm(u3, f.sub(p2, one))), f.add(f.add(pu, pu), pu));
+    // c^-1 mod p^2: inverse mod p, one Newton (Hensel) step x (2 - c x).
+    const Res<L> cm = f.to_m_signed(cc, N);
+    const uint64_t cabs = (uint64_t)(cc < 0 ? -cc : cc) % p;
+    uint64_t x0 = inv_mod64(cabs, p);
+    if (cc < 0 && x0) x0 = p - x0;
+    const Res<L> xm = f.to_m((u128)x0);
+    const Res<L> cinv = f.mm(xm, f.sub(two, f.mm(cm, xm)));
+    const Res<L> w2 = f.sub(f.mm(pm, cinv), cm);
+    const Res<L> three = f.add(two, one);
+    const u128 inv2 = (N + 1) / 2;
+    const Res<L> w3 = f.mm(f.sub(f.pow(three, p), one), f.to_m(inv2));
+    r = f.mm(f.mm(f.mm(v6, w1), w2), w3);
+  }
+  Res<L> R = f.from_m(r);
+  R = add_mod<L>(R, res_small<L>(1), f.c.n);
+  uint64_t q, rr;
+  divmod_sq(res_to_u128<L>(R), p, &q, &rr);
+  quot[i] = q;
+  rem[i] = rr;
+  if (acu) {
+    acu[2 * (size_t)i] = acu0;
+    acu[2 * (size_t)i + 1] = acu1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (extern "C", see include/clwilson.h)
+// ---------------------------------------------------------------------------
+static inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+#define DISPATCH_L(L, CALL)                                     \
+  switch (L) {                                                  \
+    case 2: { constexpr int LL = 2; CALL; } break;              \
+    case 3: { constexpr int LL = 3; CALL; } break;              \
+    case 4: { constexpr int LL = 4; CALL; } break;              \
+    case 5: { constexpr int LL = 5; CALL; } break;              \
+    default: return clw_fail(CLW_ERR_ARG, "unsupported limb count %u", (unsigned)(L)); \
+  }
+
+extern "C" int clw_setup(clw_ctx* ctx, const clw_tp_table* t, int resume) {
+  if (!ctx || !t) return clw_fail(CLW_ERR_ARG, "clw_setup: null argument");
+  if (t->count == 0) return CLW_OK;
+  const dim3 grid(ceil_div(t->count, 256)), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_setup<LL>, grid, block, 0, clw_stream(ctx), t->p, t->type, t->tp_target,
+                                          t->n_lo, t->n_hi, t->one_lo, t->one_hi, t->r2_lo, t->r2_hi, t->ninv,
+                                          t->acc_lo, t->acc_hi, t->count, resume));
+  return clw_check_launch("setup");
+}
+
+extern "C" uint32_t clw_blocks_upto(uint64_t lo, uint64_t hi, uint64_t T) {
+  const uint64_t base = lo / 30 * 30;
+  uint64_t top = hi < T + 1 ? hi : T + 1;
+  if (top <= lo) return 0;
+  return ceil_div(top - base, CLW_BLOCK_SPAN);
+}
+
+extern "C" int clw_sieve_segment(clw_ctx* ctx, const clw_seg_buffers* s, uint64_t lo, uint64_t hi,
+                                 const uint64_t targets[3], uint32_t* nblocks) {
+  if (!ctx || !s || !targets) return clw_fail(CLW_ERR_ARG, "clw_sieve_segment: null argument");
+  if (lo < 7 || hi <= lo) return clw_fail(CLW_ERR_ARG, "clw_sieve_segment: bad range [%llu, %llu)",
+                                          (unsigned long long)lo, (unsigned long long)hi);
+  const uint64_t base = lo / 30 * 30;
+  const uint64_t nb = (hi - base + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN;
+  if (nb > s->max_blocks) return clw_fail(CLW_ERR_ARG, "segment needs %llu blocks > %u", (unsigned long long)nb,
+                                          s->max_blocks);
+  const int nbases = mr_bases_needed(hi);
+  hipLaunchKernelGGL(k_sieve_pack, dim3((uint32_t)nb), dim3(256), 0, clw_stream(ctx), base, lo, hi, targets[0],
+                     targets[1], targets[2], (uint32_t)nbases, s->side_cap, s->rec, s->nrec, s->nprimes, s->side[0],
+                     s->side[1], s->side[2], s->side_count, s->flags, (unsigned long long*)s->total);
+  if (nblocks) *nblocks = (uint32_t)nb;
+  return clw_check_launch("sieve_pack");
+}
+
+extern "C" int clw_mul_records(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
+                               uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K, uint64_t* part_lo,
+                               uint64_t* part_hi, uint32_t part_stride) {
+  if (!ctx || !t || !s || type > 2 || K == 0 || tp_off + tp_n > t->count || nblk > s->max_blocks ||
+      tp_n > part_stride)
+    return clw_fail(CLW_ERR_ARG, "clw_mul_records: bad arguments");
+  if (tp_n == 0) return CLW_OK;
+  const dim3 grid(ceil_div(tp_n, 256), K), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_mul_records<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), s->rec,
+                                          s->nrec, part_lo, part_hi, tp_off, tp_n, nblk, K, part_stride, type));
+  return clw_check_launch("mul_records");
+}
+
+extern "C" int clw_mul_records_lanes(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s,
+                                     uint32_t type, uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K,
+                                     uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride) {
+  if (!ctx || !t || !s || type > 2 || K == 0 || tp_off + tp_n > t->count || nblk > s->max_blocks ||
+      tp_n > part_stride)
+    return clw_fail(CLW_ERR_ARG, "clw_mul_records_lanes: bad arguments");
+  if (tp_n == 0) return CLW_OK;
+  const dim3 grid(ceil_div(tp_n, 4), K), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_mul_records_lanes<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t),
+                                          s->rec, s->nrec, part_lo, part_hi, tp_off, tp_n, nblk, K, part_stride,
+                                          type));
+  return clw_check_launch("mul_records_lanes");
+}
+
+extern "C" int clw_mul_side(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
+                            uint32_t tp_off, uint32_t tp_n, uint32_t K, uint64_t* part_lo, uint64_t* part_hi,
+                            uint32_t part_stride) {
+  if (!ctx || !t || !s || type > 2 || K == 0 || tp_off + tp_n > t->count || tp_n > part_stride)
+    return clw_fail(CLW_ERR_ARG, "clw_mul_side: bad arguments");
+  if (tp_n == 0) return CLW_OK;
+  const dim3 grid(ceil_div(tp_n, 256), K), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_mul_side<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t),
+                                          s->side[type], s->side_count, s->side_cap, part_lo, part_hi, tp_off, tp_n,
+                                          K, part_stride, type));
+  return clw_check_launch("mul_side");
+}
+
+extern "C" int clw_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp_off, uint32_t tp_n, uint32_t nparts,
+                          const uint64_t* part_lo, const uint64_t* part_hi, uint32_t part_stride) {
+  if (!ctx || !t || tp_off + tp_n > t->count || tp_n > part_stride)
+    return clw_fail(CLW_ERR_ARG, "clw_reduce: bad arguments");
+  if (tp_n == 0) return CLW_OK;
+  const dim3 grid(ceil_div(tp_n, 256)), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_reduce<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), t->acc_lo,
+                                          t->acc_hi, part_lo, part_hi, tp_off, tp_n, nparts, part_stride));
+  return clw_check_launch("reduce");
+}
+
+extern "C" int clw_iterate(clw_ctx* ctx, const clw_tp_table* t, const uint64_t from[3]) {
+  if (!ctx || !t || !from) return clw_fail(CLW_ERR_ARG, "clw_iterate: null argument");
+  if (t->count == 0) return CLW_OK;
+  const dim3 grid(ceil_div(t->count, 4)), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_iterate<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), t->tp_target,
+                                          t->type, t->acc_lo, t->acc_hi, t->count, from[0], from[1], from[2]));
+  return clw_check_launch("iterate");
+}
+
+extern "C" int clw_finalize(clw_ctx* ctx, const clw_tp_table* t, uint64_t* quot, uint64_t* rem, int64_t* acu) {
+  if (!ctx || !t || !quot || !rem) return clw_fail(CLW_ERR_ARG, "clw_finalize: null argument");
+  if (t->count == 0) return CLW_OK;
+  const dim3 grid(ceil_div(t->count, 256)), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_finalize<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), t->p,
+                                          t->type, t->acc_lo, t->acc_hi, t->count, quot, rem, acu));
+  return clw_check_launch("finalize");
+}

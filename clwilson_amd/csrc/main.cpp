@@ -1,0 +1,259 @@
+// main.cpp -- clwilson command line (the reference's main.cpp + the result
+// file handling of cl_wilson.cpp), on top of the C ABI of libclwilson.so.
+//
+//   clwilson -p <pmin> -P <pmax>   test primes in [pmin, pmax) -> results.txt
+//   clwilson -s | --test           self test on the reference's 7 ranges
+//   clwilson -r ...                also verify against goodWilsonResults.txt
+//   clwilson -d <n> | --device <n> GPU index (default 0)
+#include <getopt.h>
+#include <inttypes.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <string>
+#include <vector>
+
+#include "clwilson.h"
+#include "putil.h"
+
+static const uint64_t kMaxP = 0xFFFFFFFFFFFFFFFFull / 4;  // cl_wilson.h:9
+static const char* kResultFile = "results.txt";
+static const char* kGoodFile = "goodWilsonResults.txt";
+
+static void help() {
+  printf("clwilson: search for Wilson primes (p-1)! = -1 (mod p^2) on an AMD Instinct GPU\n");
+  printf("Program usage:\n");
+  printf("-p #\tStarting prime to search p\n");
+  printf("-P #\tEnd prime to search P, range [-p, -P) exclusive, 5 <= -p <= p < -P <= %" PRIu64 "\n", kMaxP);
+  printf("\tRequired range is <= 10e6\n");
+  printf("-s\tPerform self test to verify proper operation of the program with the current GPU.\n");
+  printf("-r\tVerify all results (up to 2e13) where |w_p/p| < 1/50000 with known good file %s\n", kGoodFile);
+  printf("-d #\tGPU device index (default 0)\n");
+  printf("-h\tPrint this help\n");
+}
+
+struct Args {
+  uint64_t pmin = 0, pmax = 0;
+  bool test = false, result_test = false;
+  int device = 0;
+};
+
+static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quiet) {
+  // setupSearch (cl_wilson.cpp:1146-1182)
+  if (a.pmin == 0 || a.pmax == 0) {
+    printf("-p and -P arguments are required\nuse -h for help\n");
+    return 2;
+  }
+  if (a.pmin > a.pmax) {
+    printf("pmin <= pmax is required\nuse -h for help\n");
+    return 2;
+  }
+  if (a.pmax > a.pmin + 10000000) {
+    printf("range <= 10000000 is required\nuse -h for help\n");
+    return 2;
+  }
+  if (a.result_test && a.pmax > 20000000000000ull) {
+    printf("pmax <= 2e13 is required when verifying results.\nuse -h for help\n");
+    return 2;
+  }
+  if (!quiet) printf("Starting search at p: %" PRIu64 "\nStopping search at P: %" PRIu64 "\n", a.pmin, a.pmax);
+  const uint64_t cap = 10000000;
+  std::vector<uint64_t> p(cap / 10), q, r;
+  // The largest range has < 700k primes; size from the host count.
+  uint32_t tpcnt[3];
+  uint64_t targets[3];
+  if (clw_range_types(a.pmin, a.pmax, tpcnt, targets) != CLW_OK) {
+    printf("error: %s\n", clw_last_error());
+    return 1;
+  }
+  const uint64_t n = (uint64_t)tpcnt[0] + tpcnt[1] + tpcnt[2];
+  if (!n) {
+    printf("there are no primes to test in this range!\n");
+    return 1;
+  }
+  if (!quiet) {
+    printf("Testing %" PRIu64 " primes.  There are %u type 0 (1 mod 3) primes, %u type 1 (5 mod 12) primes, and %u "
+           "type 2 (11 mod 12) primes\n",
+           n, tpcnt[0], tpcnt[1], tpcnt[2]);
+    printf("Factorial targets are %" PRIu64 ", %" PRIu64 ", %" PRIu64 "\n", targets[0], targets[1], targets[2]);
+  }
+  p.resize(n);
+  q.resize(n);
+  r.resize(n);
+  clw_options opt = {0, 0, quiet ? 0u : 1u};
+  const int rc = clw_wilson_range(ctx, a.pmin, a.pmax, &opt, st, p.data(), q.data(), r.data(), n);
+  if (rc != CLW_OK) {
+    fprintf(stderr, "error: %s\n", clw_last_error());
+    printf("error: %s\n", clw_last_error());
+    return 1;
+  }
+  // results.txt: one line per (near-)Wilson prime, then the checksum
+  // (cl_wilson.cpp:657-688, 1093-1143).
+  FILE* out = fopen(kResultFile, "w");
+  if (!out) {
+    printf("Cannot open %s !!!\n", kResultFile);
+    return 1;
+  }
+  char line[128];
+  uint32_t lines = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    if (clw_format_result(p[k], q[k], line, sizeof(line))) {
+      fputs(line, out);
+      ++lines;
+      if (!a.result_test && !quiet) fputs(line, stdout);
+    }
+  }
+  if (lines == 0)
+    fprintf(out, "no results\n%016" PRIX64 "\n", st->checksum);
+  else
+    fprintf(out, "%016" PRIX64 "\n", st->checksum);
+  if (fclose(out) != 0) {
+    printf("Cannot close %s !!!\n", kResultFile);
+    return 1;
+  }
+  if (a.result_test) {
+    // cl_wilson.cpp:902-1014: every p with |w_p/p| < 1/50000 must be listed.
+    FILE* g = fopen(kGoodFile, "r");
+    if (!g) {
+      printf("Cannot open %s !!!\n", kGoodFile);
+      return 1;
+    }
+    std::vector<std::pair<uint64_t, int32_t>> good;
+    uint64_t sum = 0, gp;
+    int32_t gv;
+    char buf[256];
+    while (fgets(buf, sizeof(buf), g)) {
+      if (sscanf(buf, "%" SCNu64 " %" SCNd32, &gp, &gv) != 2) {
+        printf("Error reading %s !!!\n", kGoodFile);
+        fclose(g);
+        return 1;
+      }
+      sum += gp + (uint64_t)(int64_t)gv;
+      if (gp >= a.pmin && gp < a.pmax) good.push_back({gp, gv});
+    }
+    fclose(g);
+    if (sum != 0x6659912234F44428ull) {
+      printf("Checksum error in %s !!!\n", kGoodFile);
+      return 1;
+    }
+    size_t matched = 0;
+    for (uint64_t k = 0; k < n; ++k) {
+      const uint64_t neg = p[k] - q[k], small = q[k] <= neg ? q[k] : neg;
+      if ((long double)small / (long double)p[k] >= 1.0L / 50000.0L) continue;
+      const int32_t dq = small == q[k] ? (int32_t)q[k] : -(int32_t)neg;
+      bool ok = false;
+      for (auto& e : good)
+        if (e.first == p[k]) {
+          if (e.second != dq) {
+            printf("error: result check failed! p: %" PRIu64 " %d, result file is %d\n", p[k], dq, e.second);
+            return 1;
+          }
+          e.first = 0;
+          ok = true;
+          ++matched;
+          break;
+        }
+      if (!ok) {
+        printf("error: result check failed! p: %" PRIu64 " is not in result file!\n", p[k]);
+        return 1;
+      }
+    }
+    if (matched != good.size()) {
+      printf("error: result check failed! %zu results of the file were not found\n", good.size() - matched);
+      return 1;
+    }
+    printf("All results matched result file!\n");
+  }
+  if (!quiet) {
+    printf("Search finished in %.2f sec.\n", st->seconds);
+    printf("results %u, total power table primes generated %" PRIu64 ", checksum %016" PRIX64 "\n", st->resultcount,
+           st->totalcount, st->checksum);
+  }
+  return 0;
+}
+
+struct Kat {
+  uint64_t pmin, pmax;
+  uint32_t results;
+  uint64_t checksum, pi_maxt, lastp;
+  int64_t lastw;
+};
+
+// cl_wilson.cpp:1998-2139 run_test: the reference's seven known answers.
+static const Kat kKats[7] = {
+    {1239053554603ull, 1239053554604ull, 1, 0x00000240FAB1A752ull, 8257082014ull, 1239053554603ull, -4},
+    {1108967825921ull, 1108967825922ull, 1, 0x0000010233A2220Dull, 10956003002ull, 1108967825921ull, 12},
+    {5609877309359ull, 5609877309360ull, 1, 0x00000A344D7D0F58ull, 101542897873ull, 5609877309359ull, -6},
+    {16556218163369ull, 16556218163370ull, 1, 0x00000F0ECB80A0ABull, 147755473426ull, 16556218163369ull, 2},
+    {200, 564, 57, 0x00000000000080A3ull, 30, 563, 0},
+    {86000000, 87467200, 1, 0x0000097C61AB0943ull, 2604536, 87467099, -2},
+    {17524177394450ull, 17524177394618ull, 1, 0x00005B54B4CBBC47ull, 304620766446ull, 17524177394617ull, 256},
+};
+
+static int self_test(clw_ctx* ctx) {
+  printf("Beginning self test of 7 ranges.\n\n");
+  const time_t t0 = time(nullptr);
+  int good = 0;
+  for (int k = 0; k < 7; ++k) {
+    Args a;
+    a.pmin = kKats[k].pmin;
+    a.pmax = kKats[k].pmax;
+    clw_range_stats st;
+    const int rc = run_range(ctx, a, &st, true);
+    const bool ok = rc == 0 && st.resultcount == kKats[k].results && st.checksum == kKats[k].checksum &&
+                    st.totalcount == kKats[k].pi_maxt && st.lastp == kKats[k].lastp && st.lastw == kKats[k].lastw;
+    printf("test case %d %s. (%.2f s)\n\n", k + 1, ok ? "passed" : "failed", st.seconds);
+    good += ok;
+  }
+  printf(good == 7 ? "All test cases completed successfully!\n" : "Self test FAILED!\n");
+  printf("Elapsed time: %d sec.\n", (int)(time(nullptr) - t0));
+  return good == 7 ? 0 : 1;
+}
+
+int main(int argc, char** argv) {
+  Args a;
+  static const struct option long_opts[] = {{"device", required_argument, 0, 'd'},
+                                            {"test", no_argument, 0, 's'},
+                                            {0, 0, 0, 0}};
+  int opt, ind = -1;
+  while ((opt = getopt_long(argc, argv, "p:P:srd:h", long_opts, &ind)) != -1) {
+    int st = 0;
+    switch (opt) {
+      case 'p': st = clw_parse_uint64(&a.pmin, optarg, 5, kMaxP - 1); break;
+      case 'P': st = clw_parse_uint64(&a.pmax, optarg, 6, kMaxP); break;
+      case 's': a.test = true; break;
+      case 'r': a.result_test = true; break;
+      case 'd': a.device = atoi(optarg); break;
+      case 'h':
+      default: help(); return opt == 'h' ? 0 : 2;
+    }
+    if (st == -1) {
+      printf("%s: invalid argument -%c %s\n", argv[0], opt, optarg);
+      return 2;
+    }
+    if (st == -2) {
+      printf("%s: out of range argument -%c %s\n", argv[0], opt, optarg);
+      return 2;
+    }
+  }
+  clw_ctx* ctx = nullptr;
+  if (clw_init(a.device, &ctx) != CLW_OK) {
+    fprintf(stderr, "error: %s\n", clw_last_error());
+    return 1;
+  }
+  clw_device_info info;
+  clw_get_device_info(ctx, &info);
+  printf("GPU Info:\n  Name: \t\t%s\n  Arch: \t\t%s\n  Compute Units: \t%d\n", info.name, info.arch,
+         info.compute_units);
+  int rc;
+  if (a.test) {
+    rc = self_test(ctx);
+  } else {
+    clw_range_stats st;
+    rc = run_range(ctx, a, &st, false);
+  }
+  clw_shutdown(ctx);
+  return rc;
+}

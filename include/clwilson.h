@@ -1,0 +1,259 @@
+/*
+ * clwilson.h -- C ABI of the MI355X Wilson-prime engine (libclwilson.so).
+ *
+ * This replaces the reference's OpenCL layer: simpleCL.h/simpleCL.c (device
+ * handles, runtime kernel builds, positional kernel arguments, NDRange
+ * launches, buffer I/O, profiling) and the raw cl* calls of main.cpp and
+ * cl_wilson.cpp. Kernels are compiled offline for gfx950; every launch is a
+ * typed entry point with explicit sizes. Functions return an int status
+ * (CLW_OK = 0) and never exit the process; clw_last_error() describes the
+ * last failure on the calling thread. No torch or C++ types cross this ABI.
+ *
+ * Residues are exchanged in canonical form: the integer value V mod p^2,
+ * 0 <= V < p^2, split into two uint64 words (lo, hi) in separate arrays.
+ */
+#ifndef CLWILSON_H
+#define CLWILSON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CLW_OK 0
+#define CLW_ERR_HIP (-1)      /* HIP runtime error (message in clw_last_error) */
+#define CLW_ERR_ARG (-2)      /* invalid argument / range                       */
+#define CLW_ERR_OVERFLOW (-3) /* a device capacity was exceeded (sieve, side)   */
+#define CLW_ERR_NOMEM (-4)    /* allocation failed                              */
+#define CLW_ERR_CHECK (-5)    /* Wilson check failed: (p-1)!+1 not = 0 mod p    */
+#define CLW_ERR_STATE (-6)    /* calls out of order / unusable state            */
+#define CLW_ERR_IO (-7)       /* file I/O (results, checkpoints)                */
+
+/* Device-side error flags (clw_seg_buffers.flags[0] bits). */
+#define CLW_FLAG_SIEVE_OVERFLOW 1u /* more primes in a sieve block than fit     */
+#define CLW_FLAG_REC_OVERFLOW 2u   /* more records in a block than rec_cap      */
+#define CLW_FLAG_SIDE_OVERFLOW 4u  /* side list longer than side_cap            */
+
+/* ---- device & runtime (simpleCL.h:46-141; main.cpp:192-283) ------------- */
+
+/* Opaque device context: one HIP device, one in-order stream.
+ * Replaces sclHard {platform, context, device, queue} (simpleCL.h:46-51). */
+typedef struct clw_ctx clw_ctx;
+
+typedef struct {
+  char name[256];          /* CL_DEVICE_NAME (main.cpp:247)                   */
+  char arch[64];           /* gcnArchName, e.g. gfx950                        */
+  int32_t compute_units;   /* CL_DEVICE_MAX_COMPUTE_UNITS (main.cpp:262)      */
+  int32_t clock_khz;
+  uint64_t global_mem;     /* CL_DEVICE_GLOBAL_MEM_SIZE (main.cpp:272)        */
+  uint64_t max_alloc;      /* CL_DEVICE_MAX_MEM_ALLOC_SIZE (main.cpp:267)     */
+} clw_device_info;
+
+/* Device selection, context and queue creation: main.cpp:199-241. */
+int clw_init(int device_index, clw_ctx** out);
+/* sclReleaseClHard (simpleCL.c:579-583). */
+int clw_shutdown(clw_ctx* ctx);
+/* clGetDeviceInfo queries of main.cpp:243-275. */
+int clw_get_device_info(clw_ctx* ctx, clw_device_info* info);
+/* Replaces sclPrintErrorFlags + exit (simpleCL.c:30-350): text of the last
+ * failure on this thread ("" if none). */
+const char* clw_last_error(void);
+
+/* clCreateBuffer call sites (e.g. cl_wilson.cpp:1459-1470, 1597-1612). */
+int clw_alloc(clw_ctx* ctx, size_t bytes, void** dptr);
+/* sclReleaseMemObject (simpleCL.c:585-597); NULL-safe. */
+int clw_free(clw_ctx* ctx, void* dptr);
+/* sclWrite / sclWriteNB (simpleCL.c:718-742); blocking=0 is asynchronous on
+ * the context stream (the host buffer must stay valid until clw_finish). */
+int clw_h2d(clw_ctx* ctx, void* dst, const void* src, size_t bytes, int blocking);
+/* sclRead / sclReadNB (simpleCL.c:744-768). */
+int clw_d2h(clw_ctx* ctx, void* dst, const void* src, size_t bytes, int blocking);
+/* Fills device memory with a byte value (no counterpart: the reference uses
+ * clearresult.cl / clearacu kernels for its counters). */
+int clw_memset(clw_ctx* ctx, void* dptr, int value, size_t bytes);
+/* sclFinish (simpleCL.c:770-785): wait for all queued work. */
+int clw_finish(clw_ctx* ctx);
+
+/* Events replace clEnqueueMarker + clGetEventInfo polling (cl_wilson.cpp:357-405)
+ * and ProfilesclEnqueueKernel's START/END timestamps (simpleCL.c:501-524). */
+typedef struct clw_event clw_event;
+int clw_event_create(clw_ctx* ctx, clw_event** ev);
+int clw_event_record(clw_ctx* ctx, clw_event* ev);
+/* 1 if complete, 0 if pending, <0 on error. */
+int clw_event_query(clw_event* ev);
+int clw_event_wait(clw_event* ev);
+int clw_event_elapsed_ms(clw_event* start, clw_event* stop, float* ms);
+int clw_event_destroy(clw_event* ev);
+
+/* ---- per-test-prime table (setup.cl:11-52; cl_wilson.h:11-15) ----------- */
+
+/* Structure-of-arrays table of test primes in device memory. All arrays have
+ * `count` entries. Test primes are grouped by type (0, 1, 2); the caller
+ * fills p[] and type[]; clw_setup fills the rest. `limbs` selects the
+ * Montgomery limb schedule (2..5 limbs of 30 bits, R = 2^(30*limbs)); use
+ * clw_limbs_for(pmax). acc_lo/acc_hi hold the running canonical residues. */
+typedef struct {
+  uint64_t* p;
+  uint32_t* type;
+  uint64_t* tp_target;   /* (p-1)/6, /4, /2 by type                         */
+  uint64_t* n_lo;        /* N = p^2                                          */
+  uint64_t* n_hi;
+  uint64_t* one_lo;      /* R mod N                                          */
+  uint64_t* one_hi;
+  uint64_t* r2_lo;       /* R^2 mod N                                        */
+  uint64_t* r2_hi;
+  uint32_t* ninv;        /* -N^-1 mod 2^30                                   */
+  uint64_t* acc_lo;      /* canonical running residue                        */
+  uint64_t* acc_hi;
+  uint32_t count;
+  uint32_t limbs;
+} clw_tp_table;
+
+/* Smallest limb schedule whose Montgomery radix covers p_max^2 (2N < R). */
+int clw_limbs_for(uint64_t p_max);
+/* Allocates every array of the table for `count` test primes. */
+int clw_tp_table_alloc(clw_ctx* ctx, uint32_t count, uint32_t limbs, clw_tp_table* t);
+int clw_tp_table_free(clw_ctx* ctx, clw_tp_table* t);
+
+/* Kernel setup (setup.cl:11-52): per test prime N, -N^-1, R mod N, R^2 mod N
+ * and its own factorial target; residues set to 1 unless resume != 0. */
+int clw_setup(clw_ctx* ctx, const clw_tp_table* t, int resume);
+
+/* ---- sieving-prime segments (getsegprps.cl:211-293, cl_wilson.cpp:1293-1398) */
+
+/* Per-segment device buffers. One sieve block covers CLW_BLOCK_SPAN
+ * consecutive integers. For every block the sieve writes its primes, paired
+ * in increasing order, as 32-byte records {Q limbs[5] (30-bit), e0, e1, e2}:
+ * Q = product of the pair (or one prime), e_t = the Legendre exponent of the
+ * pair's primes in T_t! for type t (0 = not applied). Prime powers whose
+ * exponent does not fit a record go to the per-type side lists as
+ * {q, e} uint64 pairs. */
+#define CLW_BLOCK_SPAN 61440u
+#define CLW_REC_CAP 4096u     /* records per block (8192 primes)              */
+#define CLW_REC_WORDS 8u
+typedef struct {
+  uint32_t* rec;          /* max_blocks * CLW_REC_CAP * 8 words               */
+  uint32_t* nrec;         /* records per block                                */
+  uint32_t* nprimes;      /* primes per block                                 */
+  uint64_t* side[3];      /* side_cap * 2 words per type                      */
+  uint32_t* side_count;   /* [3]                                              */
+  uint32_t* flags;        /* [1] CLW_FLAG_* bits                              */
+  uint64_t* total;        /* [1] primes emitted since the last clear          */
+  uint32_t max_blocks;
+  uint32_t side_cap;
+} clw_seg_buffers;
+
+int clw_seg_alloc(clw_ctx* ctx, uint32_t max_blocks, uint32_t side_cap, clw_seg_buffers* s);
+int clw_seg_free(clw_ctx* ctx, clw_seg_buffers* s);
+/* clearresult.cl: zero side counts, flags and the prime total. */
+int clw_seg_clear(clw_ctx* ctx, const clw_seg_buffers* s);
+
+/* Segment sieve, replacing getsegprps.cl (GPU 2-PRPs above 2^32) and the CPU
+ * primesieve tables below 2^32 (cl_wilson.cpp:1316-1365): exact primes q in
+ * [lo, hi) (lo >= 7), paired into records with their Legendre exponents for
+ * the type targets T[3] (T_t = 0: type absent). Blocks start at
+ * base = floor(lo/30)*30. *nblocks receives the block count. */
+int clw_sieve_segment(clw_ctx* ctx, const clw_seg_buffers* s, uint64_t lo, uint64_t hi,
+                      const uint64_t targets[3], uint32_t* nblocks);
+
+/* Number of sieve blocks of a segment starting at lo that hold primes <= T. */
+uint32_t clw_blocks_upto(uint64_t lo, uint64_t hi, uint64_t T);
+
+/* mulsmall.cl / mullarge.cl: for the test primes [tp_off, tp_off+tp_n) of one
+ * type, the product over the records of blocks [0, nblk) split into K
+ * contiguous chunks; chunk c writes its canonical product to
+ * part_lo/hi[c * part_stride + i] (i relative to tp_off). */
+int clw_mul_records(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
+                    uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K,
+                    uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride);
+/* The same product with a wave per test prime whose 64 lanes share the
+ * records (for ranges with few test primes); grid (ceil(tp_n/4), K). */
+int clw_mul_records_lanes(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
+                          uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K,
+                          uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride);
+/* The same for the side list of `type` (count read on the device), K chunks. */
+int clw_mul_side(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
+                 uint32_t tp_off, uint32_t tp_n, uint32_t K,
+                 uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride);
+/* reduce.cl: acc[tp_off+i] *= prod_{c<nparts} part[c * part_stride + i]. */
+int clw_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp_off, uint32_t tp_n, uint32_t nparts,
+               const uint64_t* part_lo, const uint64_t* part_hi, uint32_t part_stride);
+
+/* iterate.cl: acc[i] *= prod_{n = from[type]+1}^{tp_target[i]} n for every
+ * test prime (from = the type targets). */
+int clw_iterate(clw_ctx* ctx, const clw_tp_table* t, const uint64_t from[3]);
+
+/* find.cl + processResult (cl_wilson.cpp:542-870): from acc = T_p! mod p^2
+ * compute (p-1)! + 1 = rem + p*quot (mod p^2) with the type identities; the
+ * a / c / u representations come from Cornacchia's algorithm. acu may be
+ * NULL; else acu[2*i] = u or a, acu[2*i+1] = c. */
+int clw_finalize(clw_ctx* ctx, const clw_tp_table* t, uint64_t* quot, uint64_t* rem, int64_t* acu);
+
+/* ---- whole range: cl_wilson() (cl_wilson.cpp:1442-1974) ---------------- */
+
+typedef struct {
+  uint64_t segment_span;   /* integers per sieve segment (0 = default)        */
+  uint32_t target_waves;   /* waves to aim for per product launch (0 = auto)  */
+  uint32_t verbose;        /* progress lines on stderr                        */
+} clw_options;
+
+typedef struct {
+  uint64_t pmin, pmax;
+  uint64_t checksum;       /* sum of p + rem + quot (cl_wilson.cpp:900)       */
+  uint64_t totalcount;     /* sieving primes applied = pi(maxtarget)          */
+  uint64_t typeTarget[3];
+  uint64_t maxtarget;
+  uint64_t lastp;          /* last reported (near-)Wilson prime               */
+  int64_t lastw;           /* its signed Wilson quotient                      */
+  uint32_t tpcount, tpcnt[3], resultcount, limbs;
+  double seconds;          /* wall time of the GPU phases                     */
+} clw_range_stats;
+
+/* Runs the plain method on [pmin, pmax) (5 <= pmin < pmax <= 2^62-1, width
+ * <= 10^7). If out_* are non-NULL they receive, in increasing p, the test
+ * primes, their Wilson quotients quot = ((p-1)!+1)/p mod p and rem. */
+int clw_wilson_range(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
+                     clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot, uint64_t* out_rem,
+                     uint64_t out_cap);
+
+/* The same run in phases (the structure of cl_wilson(): setup, segment loop,
+ * iterate, getResults). A session owns the device state of one range. */
+typedef struct clw_session clw_session;
+/* Test primes, typing, device table and setup (cl_wilson.cpp:1442-1843). */
+int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt, clw_session** out);
+int clw_session_destroy(clw_session* s);
+/* Counts and targets of the session (checksum / results fields are 0). */
+int clw_session_info(const clw_session* s, clw_range_stats* stats);
+/* Applies every sieving prime q in [qlo, qhi), clipped to [2, maxtarget], to
+ * the running residues (the segment loop, cl_wilson.cpp:1856-1929). Calls
+ * must cover disjoint windows; a full search covers [2, maxtarget]. */
+int clw_session_run(clw_session* s, uint64_t qlo, uint64_t qhi);
+/* Canonical running residues and the test primes, in increasing p. */
+int clw_session_residues(clw_session* s, uint64_t* lo, uint64_t* hi, uint64_t cap);
+int clw_session_primes(const clw_session* s, uint64_t* out, uint64_t cap);
+/* iterate.cl for every test prime (cl_wilson.cpp:1938-1946). */
+int clw_session_iterate(clw_session* s);
+/* getResults/processResult (cl_wilson.cpp:1017-1090, 691-900) on the GPU;
+ * fills stats (checksum, results, last result) and the per-prime outputs. */
+int clw_session_finalize(clw_session* s, clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot,
+                         uint64_t* out_rem, uint64_t out_cap);
+
+/* Result line of cl_wilson.cpp:657-688 for one test prime into buf; returns
+ * 1 if the prime is reported (quot = 0 or min(quot, p-quot) < 1000), else 0. */
+int clw_format_result(uint64_t p, uint64_t quot, char* buf, size_t buflen);
+
+/* Counts of test primes by type and the type targets of [pmin, pmax)
+ * (cl_wilson.cpp:1552-1586), computed on the host. */
+int clw_range_types(uint64_t pmin, uint64_t pmax, uint32_t tpcnt[3], uint64_t targets[3]);
+
+/* Exact primes in [lo, hi) on the host (the primesieve_generate_primes call of
+ * cl_wilson.cpp:1529). Returns the count; writes at most cap primes. */
+uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLWILSON_H */

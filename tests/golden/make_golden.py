@@ -1,0 +1,114 @@
+"""Writes the golden fixtures of tests/golden/ with the CPU oracle.
+
+    python tests/golden/make_golden.py [--cfg1]
+
+Fixtures (JSON):
+  kat_selftest.json   the reference's seven self-test answers (cl_wilson.cpp:2007-2123)
+  cgh_tables.json     Costa-Gerbicz-Harvey Tables 1-2 (w_p for |w_p| small)
+  oracle_small.json   per-prime quotients of small ranges computed by the oracle
+  windows.json        canonical window products at config-2/3 magnitudes
+  cfg1.json           (--cfg1) the whole of [1e6, 2e6): checksum, results, sha256
+                      of the Wilson quotient vector (little-endian uint64)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+import oracle as O  # noqa: E402
+
+# cl_wilson.cpp:2007-2123 -- (pmin, pmax, results, checksum, pi(maxT), last p, last w)
+KATS = [
+    (1239053554603, 1239053554604, 1, 0x00000240FAB1A752, 8257082014, 1239053554603, -4),
+    (1108967825921, 1108967825922, 1, 0x0000010233A2220D, 10956003002, 1108967825921, 12),
+    (5609877309359, 5609877309360, 1, 0x00000A344D7D0F58, 101542897873, 5609877309359, -6),
+    (16556218163369, 16556218163370, 1, 0x00000F0ECB80A0AB, 147755473426, 16556218163369, 2),
+    (200, 564, 57, 0x00000000000080A3, 30, 563, 0),
+    (86000000, 87467200, 1, 0x0000097C61AB0943, 2604536, 87467099, -2),
+    (17524177394450, 17524177394618, 1, 0x00005B54B4CBBC47, 304620766446, 17524177394617, 256),
+]
+
+# "GERBICZ a search for wilson primes.pdf" p.3073, Tables 1-2 (w_p in (-p/2, p/2)).
+CGH_T1 = [(56151923, -1), (4036677373, -5), (11774118061, -1), (5609877309359, -6), (14296621, 2),
+          (10746881, -7), (87467099, -2), (11892977, -7), (16556218163369, 2), (39198017, -7),
+          (8315831, 3), (1767839071, 8), (93559087, -3), (29085907, 9), (51802061, 4),
+          (67133912011, 9), (258818504023, 4), (42647052491, 10), (1239053554603, -4),
+          (935606702249, -10), (1987272877, 5)]
+CGH_T2 = [(5, 0), (13, 0), (563, 0), (17475368544847, 154), (13561740531809, 120),
+          (9461354987597, 94), (16556218163369, 2), (13707091918909, 143), (5609877309359, -6),
+          (935606702249, -10), (14875476519749, -38), (1108967825921, 12), (15395725531427, 46),
+          (2170161095393, 25), (1239053554603, -4), (16690620863071, 203), (4663421363459, 28),
+          (2462223083147, -35), (7746014299613, 47), (17524177394617, 256), (11273815078217, 88),
+          (10865903332033, 159), (7338481259891, -62), (16880979600449, 253)]
+
+SMALL_RANGES = [(5, 20000), (100000, 120000), (1000000, 1020000), (1999000, 2000000),
+                (86000000, 86010000)]
+
+
+def dump(name, obj):
+    with open(os.path.join(HERE, name), "w") as f:
+        json.dump(obj, f, indent=1)
+        f.write("\n")
+
+
+def quot_hash(q: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(q, dtype="<u8").tobytes()).hexdigest()
+
+
+def main():
+    dump("kat_selftest.json", [dict(pmin=a, pmax=b, results=c, checksum=f"{d:016X}", pi_maxt=e, lastp=f,
+                                    lastw=g) for a, b, c, d, e, f, g in KATS])
+    dump("cgh_tables.json", dict(table1=CGH_T1, table2=CGH_T2))
+    small = []
+    for lo, hi in SMALL_RANGES:
+        t = time.time()
+        arr, st = O.wilson_range(lo, hi)
+        assert st["rc"] == 0
+        small.append(dict(pmin=lo, pmax=hi, checksum=f"{st['checksum']:016X}", pi_maxt=st["totalcount"],
+                          results=st["resultcount"], p=[int(x) for x in arr["p"]],
+                          quot=[int(x) for x in arr["quot"]]))
+        print("small", lo, hi, round(time.time() - t, 2), flush=True)
+    dump("oracle_small.json", small)
+    # Window products at config-2 / config-3 magnitudes: a few test primes of
+    # each type, sieving primes in a fixed window of the target axis.
+    wins = []
+    for (plo, phi, qlo, qhi) in [(100000000000, 100000000400, 16666000000, 16666400000),
+                                 (20000000000000, 20000000000400, 3333333000000, 3333333300000),
+                                 (20000000000000, 20000000000400, 9999999700000, 10000000000056)]:
+        ps = O.primes_range(plo, phi)
+        T = {}
+        for p in ps:
+            p = int(p)
+            t = 0 if p % 3 == 1 else (1 if p % 12 == 5 else 2)
+            T.setdefault(t, (p - 1) // (6, 4, 2)[t])
+        typ = [0 if int(p) % 3 == 1 else (1 if int(p) % 12 == 5 else 2) for p in ps]
+        targets = np.array([T[t] for t in typ], dtype=np.uint64)
+        t0 = time.time()
+        vals = O.window_product(ps, targets, qlo, qhi)
+        print("window", plo, qlo, qhi, round(time.time() - t0, 2), flush=True)
+        wins.append(dict(pmin=plo, pmax=phi, qlo=qlo, qhi=qhi, p=[int(x) for x in ps],
+                         type_targets={str(k): v for k, v in T.items()}, value=[str(v) for v in vals]))
+    dump("windows.json", wins)
+    if "--cfg1" in sys.argv:
+        t = time.time()
+        arr, st = O.wilson_range(1000000, 2000000)
+        assert st["rc"] == 0
+        res = [dict(p=int(p), w=O.wilson_quotient(int(p), int(q))) for p, q in zip(arr["p"], arr["quot"])
+               if int(q) == 0 or min(int(q), int(p) - int(q)) < 1000]
+        dump("cfg1.json", dict(pmin=1000000, pmax=2000000, tpcount=len(arr["p"]),
+                               checksum=f"{st['checksum']:016X}", pi_maxt=st["totalcount"],
+                               type_targets=st["typeTarget"], tpcnt=st["tpcnt"],
+                               resultcount=st["resultcount"], results=res,
+                               quot_sha256=quot_hash(arr["quot"]), p_sha256=quot_hash(arr["p"])))
+        print("cfg1", round(time.time() - t, 1), hex(st["checksum"]), flush=True)
+
+
+if __name__ == "__main__":
+    main()

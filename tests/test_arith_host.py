@@ -1,0 +1,150 @@
+"""The device arithmetic headers (clw_arith.h, clw_nt64.h) compiled for the
+host CPU (libclw_hostcheck.so) and checked against Python big integers,
+sympy and the oracle's exhaustive searches."""
+import ctypes as C
+import os
+import random
+
+import pytest
+import sympy
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HC = os.path.join(ROOT, "clwilson_amd", "libclw_hostcheck.so")
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(HC):
+        from clwilson_amd import build
+        build.build_hostcheck()
+    L = C.CDLL(HC)
+    u64p = C.POINTER(C.c_uint64)
+    L.clwh_montmul.argtypes = [C.c_int, u64p, u64p, u64p, u64p]
+    L.clwh_undrift.argtypes = [C.c_int, C.c_uint64, u64p, C.c_uint64, u64p]
+    L.clwh_neg_inv30.restype = C.c_uint32
+    L.clwh_neg_inv30.argtypes = [C.c_uint32]
+    L.clwh_legendre.restype = C.c_uint64
+    L.clwh_legendre.argtypes = [C.c_uint64, C.c_uint64]
+    L.clwh_isqrt.restype = C.c_uint64
+    L.clwh_isqrt.argtypes = [C.c_uint64]
+    L.clwh_inv_mod.restype = C.c_uint64
+    L.clwh_inv_mod.argtypes = [C.c_uint64, C.c_uint64]
+    L.clwh_mr_bases_needed.argtypes = [C.c_uint64]
+    L.clwh_is_prime_mr.argtypes = [C.c_uint64, C.c_int]
+    L.clwh_two_squares_a.restype = C.c_int64
+    L.clwh_two_squares_a.argtypes = [C.c_uint64]
+    L.clwh_eisenstein_cu.argtypes = [C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    return L
+
+
+def _arr(v):
+    return (C.c_uint64 * 2)(v & (2**64 - 1), v >> 64)
+
+
+@pytest.mark.parametrize("L", [2, 3, 4, 5])
+def test_montmul_lazy_bounds(hc, L):
+    """a*b*R^-1 mod N with the result < 2N for a < 2N, b < R/2 (R = 2^(30L))."""
+    R = 2 ** (30 * L)
+    rng = random.Random(L)
+    n_ok = 0
+    while n_ok < 1500:
+        bits = rng.randint(3, min(15 * L - 1, 62))
+        p = sympy.nextprime(rng.getrandbits(bits) | 3)
+        N = p * p
+        if 2 * N >= R:
+            continue
+        a = rng.randrange(2 * N)
+        b = rng.randrange(min(R // 2, 2**128))
+        out = (C.c_uint64 * 2)()
+        hc.clwh_montmul(L, _arr(a), _arr(b), _arr(N), out)
+        r = out[0] | (out[1] << 64)
+        assert r < 2 * N and r % N == a * b * pow(R, -1, N) % N
+        # both operands < 2N (mul_full before its conditional subtraction)
+        b2 = rng.randrange(2 * N)
+        hc.clwh_montmul(L, _arr(a), _arr(b2), _arr(N), out)
+        r = out[0] | (out[1] << 64)
+        assert r < 3 * N and r % N == a * b2 * pow(R, -1, N) % N
+        n_ok += 1
+
+
+@pytest.mark.parametrize("L", [2, 3, 4, 5])
+def test_undrift(hc, L):
+    R = 2 ** (30 * L)
+    rng = random.Random(10 + L)
+    for _ in range(300):
+        bits = rng.randint(3, min(15 * L - 1, 62))
+        p = sympy.nextprime(rng.getrandbits(bits) | 3)
+        N = p * p
+        if 2 * N >= R:
+            continue
+        a = rng.randrange(2 * N)
+        d = rng.choice([0, 1, 2, rng.randrange(1000), rng.randrange(2**40)])
+        out = (C.c_uint64 * 2)()
+        hc.clwh_undrift(L, p, _arr(a), d, out)
+        assert (out[0] | (out[1] << 64)) == a * pow(R, d, N) % N
+
+
+def test_neg_inv30(hc):
+    for n0 in [1, 3, 0x3FFFFFFF, 123456789 | 1]:
+        assert (n0 * hc.clwh_neg_inv30(n0)) % 2**30 == 2**30 - 1
+
+
+def test_miller_rabin_exact(hc):
+    for n in range(39, 300000, 2):
+        assert hc.clwh_is_prime_mr(n, hc.clwh_mr_bases_needed(n)) == int(sympy.isprime(n)), n
+    rng = random.Random(1)
+    for _ in range(20000):
+        n = rng.getrandbits(rng.randint(20, 64)) | 1
+        assert hc.clwh_is_prime_mr(n, hc.clwh_mr_bases_needed(n)) == int(sympy.isprime(n)), n
+    # base-2 strong pseudoprimes are rejected by the extra bases
+    for n in (2047, 3277, 4033, 4681, 8321, 3215031751, 2152302898747, 3474749660383):
+        assert hc.clwh_is_prime_mr(n, hc.clwh_mr_bases_needed(n)) == 0
+
+
+def test_legendre_isqrt_inverse(hc):
+    rng = random.Random(2)
+    for _ in range(5000):
+        T = rng.randrange(1, 2**61)
+        q = int(sympy.prevprime(rng.randrange(3, min(T + 3, 2**40))))
+        assert hc.clwh_legendre(q, T) == O.legendre(q, T)[0]
+        n = rng.getrandbits(64)
+        r = hc.clwh_isqrt(n)
+        assert r * r <= n < (r + 1) * (r + 1)
+        p = sympy.nextprime(rng.randrange(10**6, 2**62))
+        a = rng.randrange(1, p)
+        assert a * hc.clwh_inv_mod(a, p) % p == 1
+
+
+def test_cornacchia_vs_exhaustive(hc):
+    """find_a / find_c (exhaustive, find.cl) equal Cornacchia's results."""
+    L = O.lib()
+    rng = random.Random(4)
+    ps = list(sympy.primerange(5, 30000)) + [sympy.nextprime(rng.randrange(10**6, 10**10)) for _ in range(400)]
+    for p in ps:
+        if p % 4 == 1:
+            assert hc.clwh_two_squares_a(p) == L.orc_find_a(p)
+        if p % 3 == 1:
+            c, u = C.c_int64(), C.c_int64()
+            hc.clwh_eisenstein_cu(p, C.byref(c), C.byref(u))
+            assert c.value == L.orc_find_c(p)
+            umod = 1 if ((p - 1) // 6) % 2 == 0 else 2
+            assert abs(u.value) == abs(c.value) and u.value % 3 == umod
+
+
+def test_cornacchia_large(hc):
+    """p up to 2^62: identities checked directly (exhaustive search too slow)."""
+    import math
+    rng = random.Random(6)
+    for _ in range(3000):
+        p = sympy.nextprime(rng.randrange(10**10, 2**62))
+        if p % 4 == 1:
+            a = hc.clwh_two_squares_a(p)
+            b2 = p - a * a
+            assert a % 4 == 1 and b2 > 0 and math.isqrt(b2) ** 2 == b2
+        if p % 3 == 1:
+            c, u = C.c_int64(), C.c_int64()
+            hc.clwh_eisenstein_cu(p, C.byref(c), C.byref(u))
+            d2 = 4 * p - c.value ** 2
+            assert c.value % 3 == 1 and d2 % 27 == 0 and math.isqrt(d2 // 27) ** 2 == d2 // 27
