@@ -61,8 +61,19 @@ class SegBuffers(C.Structure):
                 ("total", C.c_void_p), ("max_blocks", C.c_uint32), ("side_cap", C.c_uint32)]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("segments", C.c_uint64), ("sieve_launches", C.c_uint64), ("sieve_integers", C.c_uint64),
+                ("mul_launches", C.c_uint64), ("mul_record_apps", C.c_uint64), ("mul_prime_apps", C.c_uint64),
+                ("side_launches", C.c_uint64), ("reduce_launches", C.c_uint64), ("sieve_ms", C.c_double),
+                ("mul_ms", C.c_double), ("side_ms", C.c_double), ("reduce_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Options(C.Structure):
-    _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32)]
+    _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
+                ("mode", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -118,21 +129,35 @@ def lib():
         "clw_format_result": (i32, [u64, u64, C.c_char_p, C.c_size_t]),
         "clw_range_types": (i32, [u64, u64, P(u32), P(u64)]),
         "clw_host_primes": (u64, [u64, u64, vp, u64]),
+        "clw_mul_records_lanes": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, u32, vp, vp, u32]),
+        "clw_session_create": (i32, [vp, u64, u64, P(Options), P(vp)]),
+        "clw_session_destroy": (i32, [vp]),
+        "clw_session_info": (i32, [vp, P(RangeStats)]),
+        "clw_session_run": (i32, [vp, u64, u64]),
+        "clw_session_residues": (i32, [vp, vp, vp, u64]),
+        "clw_session_primes": (i32, [vp, vp, u64]),
+        "clw_session_iterate": (i32, [vp]),
+        "clw_session_finalize": (i32, [vp, P(RangeStats), vp, vp, vp, u64]),
+        "clw_session_kernel_stats": (i32, [vp, P(KernelStats)]),
+        "clw_session_reset_stats": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    global _bound
+    _bound = sorted(sig)
     _lib = L
     return L
 
 
-def exported_symbols() -> list[str]:
-    """Names of every function declared in include/clwilson.h."""
-    import re
-    hdr = os.path.join(os.path.dirname(_PKG), "include", "clwilson.h")
-    txt = open(hdr).read()
-    return sorted(set(re.findall(r"\b(clw_[a-z0-9_]+)\s*\(", txt)))
+_bound: list = []
+
+
+def bound_symbols() -> list[str]:
+    """Names of the C ABI functions this module has ctypes signatures for."""
+    lib()
+    return list(_bound)
 
 
 def last_error() -> str:
@@ -217,19 +242,98 @@ class Device:
                     clock_khz=d.clock_khz, global_mem=d.global_mem)
 
     def wilson_range(self, pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
-                     verbose: bool = False, allow_check_failure: bool = False) -> RangeResult:
+                     verbose: bool = False, allow_check_failure: bool = False, mode: int = 0) -> RangeResult:
         n = int(lib().clw_host_primes(pmin, pmax, None, 0))
         p = np.zeros(max(n, 1), dtype=np.uint64)
         q = np.zeros(max(n, 1), dtype=np.uint64)
         r = np.zeros(max(n, 1), dtype=np.uint64)
         st = RangeStats()
-        opt = Options(segment_span, target_waves, 1 if verbose else 0)
+        opt = Options(segment_span, target_waves, 1 if verbose else 0, mode, 0)
         rc = lib().clw_wilson_range(self.ctx, pmin, pmax, C.byref(opt), C.byref(st), p.ctypes.data,
                                     q.ctypes.data, r.ctypes.data, n)
         if rc != CLW_OK and not (allow_check_failure and rc == CLW_ERR_CHECK):
             check(rc, f"wilson_range [{pmin}, {pmax})")
         res = RangeResult(pmin, pmax, p[:n], q[:n], r[:n], st.checksum, st.totalcount, list(st.typeTarget),
                           list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs, st.seconds)
+        for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
+            line = format_result(pi, qi)
+            if line:
+                res.results.append(line)
+        return res
+
+
+class Session:
+    """One range in phases, as cl_wilson() runs it: create (setup), run
+    windows of the sieving-prime axis, iterate, finalize."""
+
+    def __init__(self, dev: "Device", pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
+                 mode: int = 0):
+        self.dev = dev
+        self.h = C.c_void_p()
+        opt = Options(segment_span, target_waves, 0, mode, 0)
+        check(lib().clw_session_create(dev.ctx, pmin, pmax, C.byref(opt), C.byref(self.h)), "session_create")
+        st = RangeStats()
+        check(lib().clw_session_info(self.h, C.byref(st)))
+        self.pmin, self.pmax = pmin, pmax
+        self.n = st.tpcount
+        self.type_targets = list(st.typeTarget)
+        self.tpcnt = list(st.tpcnt)
+        self.maxtarget = st.maxtarget
+        self.limbs = st.limbs
+
+    def close(self) -> None:
+        if self.h:
+            lib().clw_session_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def primes(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), dtype=np.uint64)
+        check(lib().clw_session_primes(self.h, out.ctypes.data, self.n))
+        return out[: self.n]
+
+    def run(self, qlo: int, qhi: int) -> None:
+        check(lib().clw_session_run(self.h, qlo, qhi), f"session_run [{qlo}, {qhi})")
+
+    def totalcount(self) -> int:
+        st = RangeStats()
+        check(lib().clw_session_info(self.h, C.byref(st)))
+        return st.totalcount
+
+    def residues(self) -> list[int]:
+        lo = np.zeros(max(self.n, 1), dtype=np.uint64)
+        hi = np.zeros(max(self.n, 1), dtype=np.uint64)
+        check(lib().clw_session_residues(self.h, lo.ctypes.data, hi.ctypes.data, self.n))
+        return [int(a) | (int(b) << 64) for a, b in zip(lo[: self.n], hi[: self.n])]
+
+    def iterate(self) -> None:
+        check(lib().clw_session_iterate(self.h), "session_iterate")
+
+    def kernel_stats(self) -> dict:
+        ks = KernelStats()
+        check(lib().clw_session_kernel_stats(self.h, C.byref(ks)))
+        return ks.as_dict()
+
+    def reset_stats(self) -> None:
+        check(lib().clw_session_reset_stats(self.h))
+
+    def finalize(self, allow_check_failure: bool = False) -> RangeResult:
+        n = self.n
+        p = np.zeros(max(n, 1), dtype=np.uint64)
+        q = np.zeros(max(n, 1), dtype=np.uint64)
+        r = np.zeros(max(n, 1), dtype=np.uint64)
+        st = RangeStats()
+        rc = lib().clw_session_finalize(self.h, C.byref(st), p.ctypes.data, q.ctypes.data, r.ctypes.data, n)
+        if rc != CLW_OK and not (allow_check_failure and rc == CLW_ERR_CHECK):
+            check(rc, "session_finalize")
+        res = RangeResult(self.pmin, self.pmax, p[:n], q[:n], r[:n], st.checksum, st.totalcount,
+                          list(st.typeTarget), list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs,
+                          st.seconds)
         for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
             line = format_result(pi, qi)
             if line:

@@ -169,14 +169,38 @@ struct clw_session {
   uint64_t total_primes = 0;  // sieving primes applied so far
   bool iterated = false;
   double gpu_seconds = 0;
+  // Events around the launches of one segment: sieve, then per type
+  // (records, side, reduce).
+  hipEvent_t ev[2 + 3 * 4] = {};
+  bool ev_ok = false;
+  clw_kernel_stats ks{};
 
   ~clw_session() {
+    if (ev_ok)
+      for (hipEvent_t e : ev) hipEventDestroy(e);
     clw_tp_table_free(ctx, &tab);
     clw_seg_free(ctx, &seg);
     clw_free(ctx, part_lo);
     clw_free(ctx, part_hi);
   }
 };
+
+extern "C" int clw_session_kernel_stats(const clw_session* S, clw_kernel_stats* out) {
+  if (!S || !out) return clw_fail(CLW_ERR_ARG, "clw_session_kernel_stats: null argument");
+  *out = S->ks;
+  return CLW_OK;
+}
+
+extern "C" int clw_session_reset_stats(clw_session* S) {
+  if (!S) return clw_fail(CLW_ERR_ARG, "clw_session_reset_stats: null session");
+  memset(&S->ks, 0, sizeof(S->ks));
+  return CLW_OK;
+}
+
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.0f;
+}
 
 extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
                                   clw_session** out) {
@@ -190,7 +214,7 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
     return clw_fail(CLW_ERR_ARG, "range width %llu > 10000000", (unsigned long long)(pmax - pmin));
   clw_session* S = new clw_session();
   S->ctx = ctx;
-  S->opt = opt ? *opt : clw_options{0, 0, 0};
+  S->opt = opt ? *opt : clw_options{0, 0, 0, 0, 0};
   S->pmin = pmin;
   S->pmax = pmax;
   S->tp.resize(clw_host_primes(pmin, pmax, nullptr, 0));
@@ -222,7 +246,7 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
   S->stride = std::max(S->cnt[0], std::max(S->cnt[1], S->cnt[2]));
   // Few test primes: a wave per prime (clw_mul_records_lanes) and longer
   // segments; many: a lane per prime (clw_mul_records).
-  S->lanes_mode = S->stride < kLanesModeBelow;
+  S->lanes_mode = S->opt.mode == 2 || (S->opt.mode == 0 && S->stride < kLanesModeBelow);
   S->span = S->opt.segment_span
                 ? (S->opt.segment_span + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN * CLW_BLOCK_SPAN
                 : (S->lanes_mode ? kLanesSpan : kDefaultSpan);
@@ -239,6 +263,12 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
   if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_lo);
   if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_hi);
   if (rc == CLW_OK) rc = clw_finish(ctx);
+  if (rc == CLW_OK) {
+    S->ev_ok = true;
+    for (hipEvent_t& e : S->ev)
+      if (hipEventCreate(&e) != hipSuccess) S->ev_ok = false;
+    if (!S->ev_ok) rc = clw_fail(CLW_ERR_HIP, "hipEventCreate failed");
+  }
   if (rc != CLW_OK) {
     delete S;
     return rc;
@@ -305,9 +335,15 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       }
       CK(clw_h2d(ctx, S->seg.side_count, h_side_cnt, 16, 1));
     }
+    hipStream_t st = clw_stream(ctx);
+    bool ran_sieve = false, ran_t[3] = {false, false, false}, ran_rec[3] = {false, false, false};
+    uint32_t nblk_t[3] = {0, 0, 0};
     if (have_range) {
       uint32_t nb = 0;
+      hipEventRecord(S->ev[0], st);
       CK(clw_sieve_segment(ctx, &S->seg, lo, hi, S->T, &nb));
+      hipEventRecord(S->ev[1], st);
+      ran_sieve = true;
     }
     for (uint32_t t = 0; t < 3; ++t) {
       const uint32_t n = S->cnt[t];
@@ -315,6 +351,11 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       const bool recs = have_range && S->T[t] >= lo;
       if (!recs && !(first && !small_side[t].empty())) continue;
       const uint32_t nblk = recs ? clw_blocks_upto(lo, hi, S->T[t]) : 0;
+      hipEvent_t* e4 = S->ev + 2 + 4 * t;
+      ran_t[t] = true;
+      ran_rec[t] = nblk > 0;
+      nblk_t[t] = nblk;
+      hipEventRecord(e4[0], st);
       uint32_t nparts = 0, Ks = 1;
       if (S->lanes_mode) {
         // a wave per test prime, lanes over the records; ~2+ blocks per chunk
@@ -335,10 +376,13 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
           nparts = K;
         }
       }
+      hipEventRecord(e4[1], st);
       CK(clw_mul_side(ctx, &S->tab, &S->seg, t, S->off[t], n, Ks, S->part_lo + (size_t)nparts * n,
                       S->part_hi + (size_t)nparts * n, n));
+      hipEventRecord(e4[2], st);
       nparts += Ks;
       CK(clw_reduce(ctx, &S->tab, S->off[t], n, nparts, S->part_lo, S->part_hi, n));  // reduce.cl
+      hipEventRecord(e4[3], st);
     }
     // clearn.cl: add the segment's prime count; check the device flags.
     uint32_t flags[4] = {0, 0, 0, 0};
@@ -349,6 +393,35 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded in segment [%llu, %llu): flags %u",
                       (unsigned long long)lo, (unsigned long long)hi, flags[0]);
     S->total_primes += seg_total[0];
+    // Kernel timings and work of this segment.
+    S->ks.segments++;
+    if (ran_sieve) {
+      S->ks.sieve_launches++;
+      S->ks.sieve_ms += ev_ms(S->ev[0], S->ev[1]);
+      S->ks.sieve_integers += hi - lo;
+    }
+    uint32_t maxblk = std::max(nblk_t[0], std::max(nblk_t[1], nblk_t[2]));
+    std::vector<uint32_t> nrec_h(maxblk), npr_h(maxblk);
+    if (maxblk) {
+      CK(clw_d2h(ctx, nrec_h.data(), S->seg.nrec, (size_t)maxblk * 4, 0));
+      CK(clw_d2h(ctx, npr_h.data(), S->seg.nprimes, (size_t)maxblk * 4, 1));
+    }
+    for (int t = 0; t < 3; ++t) {
+      if (!ran_t[t]) continue;
+      hipEvent_t* e4 = S->ev + 2 + 4 * t;
+      if (ran_rec[t]) {
+        S->ks.mul_launches++;
+        S->ks.mul_ms += ev_ms(e4[0], e4[1]);
+        uint64_t recs = 0, prs = 0;
+        for (uint32_t b = 0; b < nblk_t[t]; ++b) recs += nrec_h[b], prs += npr_h[b];
+        S->ks.mul_record_apps += recs * S->cnt[t];
+        S->ks.mul_prime_apps += prs * S->cnt[t];
+      }
+      S->ks.side_launches++;
+      S->ks.side_ms += ev_ms(e4[1], e4[2]);
+      S->ks.reduce_launches++;
+      S->ks.reduce_ms += ev_ms(e4[2], e4[3]);
+    }
     first = false;
     if (!have_range) break;
     lo = hi;

@@ -82,7 +82,7 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   p.resize(n);
   q.resize(n);
   r.resize(n);
-  clw_options opt = {0, 0, quiet ? 0u : 1u};
+  clw_options opt = {0, 0, quiet ? 0u : 1u, 0, 0};
   const int rc = clw_wilson_range(ctx, a.pmin, a.pmax, &opt, st, p.data(), q.data(), r.data(), n);
   if (rc != CLW_OK) {
     fprintf(stderr, "error: %s\n", clw_last_error());

@@ -198,6 +198,10 @@ typedef struct {
   uint64_t segment_span;   /* integers per sieve segment (0 = default)        */
   uint32_t target_waves;   /* waves to aim for per product launch (0 = auto)  */
   uint32_t verbose;        /* progress lines on stderr                        */
+  uint32_t mode;           /* product kernel: 0 auto, 1 lane per test prime
+                              (clw_mul_records), 2 wave per test prime
+                              (clw_mul_records_lanes)                         */
+  uint32_t reserved;
 } clw_options;
 
 typedef struct {
@@ -240,6 +244,19 @@ int clw_session_iterate(clw_session* s);
  * fills stats (checksum, results, last result) and the per-prime outputs. */
 int clw_session_finalize(clw_session* s, clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot,
                          uint64_t* out_rem, uint64_t out_cap);
+
+/* Per-kernel timing of a session, from HIP events recorded around every
+ * launch of clw_session_run (no counterpart in the reference beyond
+ * ProfilesclEnqueueKernel). record_apps = records x test primes applied. */
+typedef struct {
+  uint64_t segments;
+  uint64_t sieve_launches, sieve_integers;
+  uint64_t mul_launches, mul_record_apps, mul_prime_apps;
+  uint64_t side_launches, reduce_launches;
+  double sieve_ms, mul_ms, side_ms, reduce_ms;
+} clw_kernel_stats;
+int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
+int clw_session_reset_stats(clw_session* s);
 
 /* Result line of cl_wilson.cpp:657-688 for one test prime into buf; returns
  * 1 if the prime is reported (quot = 0 or min(quot, p-quot) < 1000), else 0. */

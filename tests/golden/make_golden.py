@@ -62,6 +62,58 @@ def quot_hash(q: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(q, dtype="<u8").tobytes()).hexdigest()
 
 
+def _two_squares_odd(p):
+    """a with a^2 + b^2 = p, a odd, a = 1 (mod 4) (find.cl:46-65 normalisation)."""
+    import math
+    z = 2
+    while pow(z, (p - 1) // 2, p) != p - 1:
+        z += 1
+    x = pow(z, (p - 1) // 4, p)
+    r0, r1 = p, x
+    while r1 * r1 > p:
+        r0, r1 = r1, r0 % r1
+    a, b = r1, math.isqrt(p - r1 * r1)
+    assert a * a + b * b == p
+    odd = a if a % 2 else b
+    return -odd if odd % 4 == 3 else odd
+
+
+def _eisenstein_c(p):
+    """c with c^2 + 27 d^2 = 4p, c = 1 (mod 3) (find.cl:68-88 normalisation)."""
+    import math
+    g = 2
+    while pow(g, (p - 1) // 3, p) == 1:
+        g += 1
+    w = pow(g, (p - 1) // 3, p)
+    s = (2 * w + 1) % p
+    if s % 2 == 0:
+        s = p - s
+    r0, r1 = 2 * p, s
+    while r1 * r1 >= 4 * p:
+        r0, r1 = r1, r0 % r1
+    x, y = r1, math.isqrt((4 * p - r1 * r1) // 3)
+    assert x * x + 3 * y * y == 4 * p
+    for u, v in ((x, y), ((x - 3 * y) // 2, (x + y) // 2), ((x + 3 * y) // 2, (x - y) // 2)):
+        if v % 3 == 0:
+            c = abs(u)
+            assert c * c + 27 * (v // 3) ** 2 == 4 * p
+            return -c if c % 3 == 2 else c
+    raise AssertionError(p)
+
+
+def acu_values(p, t):
+    """(acu1, acu2) for the oracle's finalize: (u, c) for type 0 (u = +-c with
+    u = umod (mod 3), one of the valid u), (a, 0) for type 1."""
+    if t == 1:
+        return _two_squares_odd(p), 0
+    if t == 0:
+        c = _eisenstein_c(p)
+        umod = 1 if ((p - 1) // 6) % 2 == 0 else 2
+        u = abs(c) if abs(c) % 3 == umod else -abs(c)
+        return u, c
+    return 0, 0
+
+
 def main():
     dump("kat_selftest.json", [dict(pmin=a, pmax=b, results=c, checksum=f"{d:016X}", pi_maxt=e, lastp=f,
                                     lastw=g) for a, b, c, d, e, f, g in KATS])
@@ -81,7 +133,10 @@ def main():
     wins = []
     for (plo, phi, qlo, qhi) in [(100000000000, 100000000400, 16666000000, 16666400000),
                                  (20000000000000, 20000000000400, 3333333000000, 3333333300000),
-                                 (20000000000000, 20000000000400, 9999999700000, 10000000000056)]:
+                                 (20000000000000, 20000000000400, 9999999700000, 10000000000056),
+                                 (1000000000000000, 1000000000000300, 166666666500000, 166666666700000),
+                                 (1000000000000000000, 1000000000000000200, 166666666666600000,
+                                  166666666666700000)]:
         ps = O.primes_range(plo, phi)
         T = {}
         for p in ps:
@@ -93,8 +148,15 @@ def main():
         t0 = time.time()
         vals = O.window_product(ps, targets, qlo, qhi)
         print("window", plo, qlo, qhi, round(time.time() - t0, 2), flush=True)
+        own = np.array([(int(p) - 1) // (6, 4, 2)[t] for p, t in zip(ps, typ)], dtype=np.uint64)
+        itv = O.iterate(ps, targets, own, vals)
+        fin = []
+        for p, t, v in zip(ps, typ, itv):
+            acu1, acu2 = acu_values(int(p), t)
+            fin.append(list(O.finalize(int(p), t, v, acu1, acu2)))
         wins.append(dict(pmin=plo, pmax=phi, qlo=qlo, qhi=qhi, p=[int(x) for x in ps],
-                         type_targets={str(k): v for k, v in T.items()}, value=[str(v) for v in vals]))
+                         type_targets={str(k): v for k, v in T.items()}, value=[str(v) for v in vals],
+                         iterated=[str(v) for v in itv], finalized=fin))
     dump("windows.json", wins)
     if "--cfg1" in sys.argv:
         t = time.time()
