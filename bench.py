@@ -124,6 +124,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", type=int, default=0)
+    ap.add_argument("--per-lane", type=int, default=0, help="test primes per lane (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would search its own "
@@ -135,7 +136,7 @@ def main() -> int:
     dev = cw.Device(0)
     info = dev.info()
     t_setup = time.time()
-    sess = cw.Session(dev, PMIN, PMAX, mode=args.mode)
+    sess = cw.Session(dev, PMIN, PMAX, mode=args.mode, per_lane=args.per_lane)
     T = sess.type_targets
     cnt = sess.tpcnt
     regs = regions(T, cnt)
@@ -217,6 +218,7 @@ def main() -> int:
             "iterate_finalize_s": t_tail,
             "projected_full_search_s": total,
             "mode": "lane per test prime" if args.mode in (0, 1) else "wave per test prime",
+            "per_lane": args.per_lane or int(cw.lib().clw_default_per_lane(L)),
         },
         "roofline": {
             "bound": "valu-int32-mad",

@@ -73,7 +73,7 @@ class KernelStats(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
-                ("mode", C.c_uint32), ("reserved", C.c_uint32)]
+                ("mode", C.c_uint32), ("per_lane", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -120,7 +120,8 @@ def lib():
         "clw_seg_clear": (i32, [vp, P(SegBuffers)]),
         "clw_sieve_segment": (i32, [vp, P(SegBuffers), u64, u64, P(u64), P(u32)]),
         "clw_blocks_upto": (u32, [u64, u64, u64]),
-        "clw_mul_records": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, u32, vp, vp, u32]),
+        "clw_mul_records": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, u32, vp, vp, u32, u32]),
+        "clw_default_per_lane": (u32, [u32]),
         "clw_mul_side": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, vp, vp, u32]),
         "clw_reduce": (i32, [vp, P(TpTable), u32, u32, u32, vp, vp, u32]),
         "clw_iterate": (i32, [vp, P(TpTable), P(u64)]),
@@ -242,13 +243,14 @@ class Device:
                     clock_khz=d.clock_khz, global_mem=d.global_mem)
 
     def wilson_range(self, pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
-                     verbose: bool = False, allow_check_failure: bool = False, mode: int = 0) -> RangeResult:
+                     verbose: bool = False, allow_check_failure: bool = False, mode: int = 0,
+                     per_lane: int = 0) -> RangeResult:
         n = int(lib().clw_host_primes(pmin, pmax, None, 0))
         p = np.zeros(max(n, 1), dtype=np.uint64)
         q = np.zeros(max(n, 1), dtype=np.uint64)
         r = np.zeros(max(n, 1), dtype=np.uint64)
         st = RangeStats()
-        opt = Options(segment_span, target_waves, 1 if verbose else 0, mode, 0)
+        opt = Options(segment_span, target_waves, 1 if verbose else 0, mode, per_lane)
         rc = lib().clw_wilson_range(self.ctx, pmin, pmax, C.byref(opt), C.byref(st), p.ctypes.data,
                                     q.ctypes.data, r.ctypes.data, n)
         if rc != CLW_OK and not (allow_check_failure and rc == CLW_ERR_CHECK):
@@ -267,10 +269,10 @@ class Session:
     windows of the sieving-prime axis, iterate, finalize."""
 
     def __init__(self, dev: "Device", pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
-                 mode: int = 0):
+                 mode: int = 0, per_lane: int = 0):
         self.dev = dev
         self.h = C.c_void_p()
-        opt = Options(segment_span, target_waves, 0, mode, 0)
+        opt = Options(segment_span, target_waves, 0, mode, per_lane)
         check(lib().clw_session_create(dev.ctx, pmin, pmax, C.byref(opt), C.byref(self.h)), "session_create")
         st = RangeStats()
         check(lib().clw_session_info(self.h, C.byref(st)))

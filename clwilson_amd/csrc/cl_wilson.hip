@@ -367,12 +367,14 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
           nparts = K;
         }
       } else {
-        const uint32_t tiles = (n + 255) / 256;
+        const uint32_t per_lane = S->opt.per_lane ? S->opt.per_lane : clw_default_per_lane(S->tab.limbs);
+        const uint32_t tiles = (n + 256 * per_lane - 1) / (256 * per_lane);
         uint32_t K = std::max<uint32_t>(1, (S->target_waves + 4 * tiles - 1) / (4 * tiles));
         K = std::min<uint32_t>(K, std::min<uint32_t>(kMaxKRec, std::max<uint32_t>(nblk, 1)));
         Ks = std::min<uint32_t>(kMaxKSide, std::max<uint32_t>(1, K / 4));
         if (nblk) {
-          CK(clw_mul_records(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n));
+          CK(clw_mul_records(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n,
+                             per_lane));
           nparts = K;
         }
       }

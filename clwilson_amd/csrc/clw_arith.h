@@ -90,6 +90,72 @@ CLW_HD Res<L> montmul(const Res<L>& a, const Res<L>& b, const Res<L>& n, uint32_
   return r;
 }
 
+// Keeps the compiler from reassociating a column sum into several partial
+// chains (each extra chain costs a 64-bit add); independent products of
+// different test primes are interleaved instead.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CLW_PIN1(a) asm volatile("" : "+v"(a))
+#else
+#define CLW_PIN1(a) ((void)0)
+#endif
+
+template <int K>
+CLW_HD void pin_all(uint64_t (&c)[K]) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) CLW_PIN1(c[j]);
+}
+
+// K independent Montgomery products r[j] = a[j] * b * R^-1 mod n[j] with a
+// common (wave-uniform) multiplier b, computed column by column in lockstep:
+// consecutive instructions belong to different test primes, so the 64-bit
+// accumulator chains never stall on each other and need no extra adds.
+template <int L, int K>
+CLW_HD void montmul_lockstep(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, const Res<L> (&n)[K],
+                             const uint32_t (&ninv)[K]) {
+  uint32_t m[K][L];
+  uint64_t c[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) c[j] = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * L - 1; ++k) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int jj = k - i;
+      if (jj >= 0 && jj < L) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) c[j] = mad32(a[j].v[i], b.v[jj], c[j]);
+        pin_all<K>(c);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int jj = k - i;
+      if (i < k && jj >= 0 && jj < L) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) c[j] = mad32(m[j][i], n[j].v[jj], c[j]);
+        pin_all<K>(c);
+      }
+    }
+    if (k < L) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        m[j][k] = ((uint32_t)c[j] * ninv[j]) & LMASK;
+        c[j] = mad32(m[j][k], n[j].v[0], c[j]) >> LB;
+      }
+      pin_all<K>(c);
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        r[j].v[k - L] = (uint32_t)c[j] & LMASK;
+        c[j] >>= LB;
+      }
+      pin_all<K>(c);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) r[j].v[L - 1] = (uint32_t)c[j];
+}
+
 // r - n if r >= n, else r (limb-wise borrow).
 template <int L>
 CLW_HD Res<L> cond_sub(const Res<L>& r, const Res<L>& n) {

@@ -165,10 +165,15 @@ uint32_t clw_blocks_upto(uint64_t lo, uint64_t hi, uint64_t T);
 /* mulsmall.cl / mullarge.cl: for the test primes [tp_off, tp_off+tp_n) of one
  * type, the product over the records of blocks [0, nblk) split into K
  * contiguous chunks; chunk c writes its canonical product to
- * part_lo/hi[c * part_stride + i] (i relative to tp_off). */
+ * part_lo/hi[c * part_stride + i] (i relative to tp_off). per_lane test
+ * primes share one lane (grid (ceil(tp_n / (256 per_lane)), K)). */
 int clw_mul_records(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
                     uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K,
-                    uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride);
+                    uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride, uint32_t per_lane);
+/* Test primes per lane of clw_mul_records for a limb schedule (1, 2 or 4;
+ * the product kernel interleaves their Montgomery products). 0 in
+ * clw_mul_records / clw_options means this default. */
+uint32_t clw_default_per_lane(uint32_t limbs);
 /* The same product with a wave per test prime whose 64 lanes share the
  * records (for ranges with few test primes); grid (ceil(tp_n/4), K). */
 int clw_mul_records_lanes(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
@@ -201,7 +206,7 @@ typedef struct {
   uint32_t mode;           /* product kernel: 0 auto, 1 lane per test prime
                               (clw_mul_records), 2 wave per test prime
                               (clw_mul_records_lanes)                         */
-  uint32_t reserved;
+  uint32_t per_lane;       /* test primes per lane in clw_mul_records (0 auto) */
 } clw_options;
 
 typedef struct {
