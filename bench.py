@@ -116,6 +116,34 @@ def cpu_baseline(T, cnt, tp, seconds_budget: float) -> dict:
                 factor_apps_per_s={r[0]: r[1] for r in rates})
 
 
+def measure_plain(dev, cw, regs, nxt, steps):
+    """The reference's launch structure on the same workload: one test prime
+    per launch pair (mullarge + reduce), one window per region."""
+    out = {}
+    with cw.Session(dev, PMIN, PMAX, mode=3) as s:
+        rates = []
+        for ri in range(min(steps, 3)):
+            hi = nxt[ri]
+            lo = hi - SPAN
+            nxt[ri] = lo
+            before = s.kernel_stats()
+            t0 = time.perf_counter()
+            s.run(lo, hi)
+            cw.lib().clw_finish(dev.ctx)
+            dt = time.perf_counter() - t0
+            apps = s.kernel_stats()["mul_prime_apps"] - before["mul_prime_apps"]
+            rates.append(apps / dt)
+            print(f"# plain region {regs[ri][0]} {dt * 1e3:.1f} ms {apps / dt:.4g} factor apps/s",
+                  file=sys.stderr, flush=True)
+        total = sum(F / rates[k] for k, (_, _, _, _, F) in enumerate(regs[:len(rates)]))
+        if len(rates) < 3:
+            mean = sum(rates) / len(rates)
+            total += sum(F / mean for (_, _, _, _, F) in regs[len(rates):])
+        out = dict(value=327107 / total, region_rates_factor_apps_per_s=rates,
+                   structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +153,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", type=int, default=0)
     ap.add_argument("--per-lane", type=int, default=0, help="test primes per lane (0 = library default)")
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="windows timed on the one-prime-per-launch path for the batched/plain ratio")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would search its own "
@@ -179,6 +209,9 @@ def main() -> int:
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t_all
     ks = sess.kernel_stats()
+    plain = None
+    if args.plain_steps > 0:
+        plain = measure_plain(dev, cw, regs, nxt, args.plain_steps)
     # iterate + final check once (their share of a full search)
     t0 = time.perf_counter()
     sess.iterate()
@@ -233,6 +266,8 @@ def main() -> int:
             "mean_launch_ms": ks["mul_ms"] / max(1, ks["mul_launches"]),
         },
         "kernel_ms": {k: ks[k] for k in ("sieve_ms", "mul_ms", "side_ms", "reduce_ms")},
+        "plain": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
         "wall_s": wall,
     }
     if not args.no_cpu:

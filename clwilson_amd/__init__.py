@@ -71,6 +71,12 @@ class KernelStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class WorkStatus(C.Structure):
+    _fields_ = [("pmin", C.c_uint64), ("pmax", C.c_uint64), ("currp", C.c_uint64), ("trickle", C.c_uint64),
+                ("state_sum", C.c_uint64), ("totalcount", C.c_uint64), ("tpcount", C.c_uint32),
+                ("done", C.c_uint32)]
+
+
 class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
                 ("mode", C.c_uint32), ("per_lane", C.c_uint32)]
@@ -141,6 +147,11 @@ def lib():
         "clw_session_finalize": (i32, [vp, P(RangeStats), vp, vp, vp, u64]),
         "clw_session_kernel_stats": (i32, [vp, P(KernelStats)]),
         "clw_session_reset_stats": (i32, [vp]),
+        "clw_ckpt_write": (i32, [C.c_char_p, P(WorkStatus), vp, vp]),
+        "clw_ckpt_read": (i32, [C.c_char_p, C.c_char_p, u64, u64, u32, P(WorkStatus), vp, vp, P(i32), P(i32)]),
+        "clw_session_set_residues": (i32, [vp, vp, vp, u64]),
+        "clw_plain_mul": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, vp, vp]),
+        "clw_plain_reduce": (i32, [vp, P(TpTable), u32, u32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -316,6 +327,19 @@ class Session:
     def iterate(self) -> None:
         check(lib().clw_session_iterate(self.h), "session_iterate")
 
+    def set_residues(self, values: list[int], totalcount: int) -> None:
+        lo = np.array([v & (2**64 - 1) for v in values], dtype=np.uint64)
+        hi = np.array([v >> 64 for v in values], dtype=np.uint64)
+        check(lib().clw_session_set_residues(self.h, lo.ctypes.data, hi.ctypes.data, totalcount),
+              "session_set_residues")
+
+    def checkpoint(self, path: str, currp: int, done: int = 0, trickle: int = 0) -> WorkStatus:
+        """Writes the reference's checkpoint layout for this session."""
+        vals = self.residues()
+        st = WorkStatus(self.pmin, self.pmax, currp, trickle, 0, self.totalcount(), self.n, done)
+        ckpt_write(path, st, vals)
+        return st
+
     def kernel_stats(self) -> dict:
         ks = KernelStats()
         check(lib().clw_session_kernel_stats(self.h, C.byref(ks)))
@@ -341,6 +365,24 @@ class Session:
             if line:
                 res.results.append(line)
         return res
+
+
+def ckpt_write(path: str, st: WorkStatus, values: list[int]) -> None:
+    lo = np.array([v & (2**64 - 1) for v in values] or [0], dtype=np.uint64)
+    hi = np.array([v >> 64 for v in values] or [0], dtype=np.uint64)
+    check(lib().clw_ckpt_write(path.encode(), C.byref(st), lo.ctypes.data, hi.ctypes.data), "ckpt_write")
+
+
+def ckpt_read(path_a: str, path_b: str, pmin: int, pmax: int, tpcount: int):
+    """(result, which, WorkStatus, residues): result 0 none, 1 resume, 2 done."""
+    st = WorkStatus()
+    lo = np.zeros(max(tpcount, 1), dtype=np.uint64)
+    hi = np.zeros(max(tpcount, 1), dtype=np.uint64)
+    which, result = C.c_int(-1), C.c_int(0)
+    check(lib().clw_ckpt_read(path_a.encode(), path_b.encode(), pmin, pmax, tpcount, C.byref(st), lo.ctypes.data,
+                              hi.ctypes.data, C.byref(which), C.byref(result)), "ckpt_read")
+    vals = [int(a) | (int(b) << 64) for a, b in zip(lo[:tpcount], hi[:tpcount])]
+    return result.value, which.value, st, vals
 
 
 def wilson_quotient(p: int, quot: int) -> int:

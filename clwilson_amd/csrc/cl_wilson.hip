@@ -26,6 +26,7 @@ const uint32_t kSideCap = 1u << 22;
 const uint32_t kMaxKRec = 32, kMaxKSide = 8, kMaxKLanes = 8192;
 const uint32_t kLanesModeBelow = 8192;              // test primes per type
 const uint64_t kLanesSpan = 16ull * kDefaultSpan;   // ~4.3e9 integers per segment
+const uint32_t kPlainGroups = 1024;                 // work-groups per plain launch
 
 // Odd primes below `lim` (simple sieve; lim is at most a few million).
 std::vector<uint32_t> small_odd_primes(uint32_t lim) {
@@ -258,8 +259,9 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
   if (rc == CLW_OK) rc = clw_seg_alloc(ctx, (uint32_t)(S->span / CLW_BLOCK_SPAN + 1), kSideCap, &S->seg);
   // Chunk products of type t live at part[c * cnt[t] + i]: at most
   // (kMaxKRec + kMaxKSide) * cnt (lane per prime) or target_waves + 2 cnt.
-  const size_t part_entries = std::max<size_t>((size_t)(kMaxKRec + kMaxKSide) * S->stride,
-                                               (size_t)S->target_waves + 2 * (size_t)S->stride);
+  const size_t part_entries = std::max<size_t>(
+      std::max<size_t>((size_t)(kMaxKRec + kMaxKSide) * S->stride, (size_t)S->target_waves + 2 * (size_t)S->stride),
+      kPlainGroups);
   if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_lo);
   if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_hi);
   if (rc == CLW_OK) rc = clw_finish(ctx);
@@ -357,7 +359,16 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       nblk_t[t] = nblk;
       hipEventRecord(e4[0], st);
       uint32_t nparts = 0, Ks = 1;
-      if (S->lanes_mode) {
+      if (S->opt.mode == 3) {
+        // one test prime per launch pair, as the reference's segment loop
+        // (cl_wilson.cpp:1895-1923) enqueues mullarge + reduce per prime
+        const uint32_t groups = std::max<uint32_t>(1, std::min<uint32_t>(kPlainGroups, nblk));
+        if (nblk)
+          for (uint32_t i = 0; i < n; ++i) {
+            CK(clw_plain_mul(ctx, &S->tab, &S->seg, t, S->off[t] + i, nblk, groups, S->part_lo, S->part_hi));
+            CK(clw_plain_reduce(ctx, &S->tab, S->off[t] + i, groups, S->part_lo, S->part_hi));
+          }
+      } else if (S->lanes_mode) {
         // a wave per test prime, lanes over the records; ~2+ blocks per chunk
         uint32_t K = std::max<uint32_t>(1, (S->target_waves + n - 1) / n);
         K = std::min<uint32_t>(K, std::max<uint32_t>(1, nblk / 2));
@@ -528,4 +539,105 @@ extern "C" int clw_wilson_range(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, cons
   if (rc == CLW_OK) rc = clw_session_finalize(S, stats, out_p, out_quot, out_rem, out_cap);
   clw_session_destroy(S);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoints (cl_wilson.cpp:110-319)
+// ---------------------------------------------------------------------------
+static uint64_t ckpt_sum(const clw_work_status* st, const uint64_t* lo, const uint64_t* hi) {
+  uint64_t s = st->pmin + st->pmax + st->currp + st->trickle + st->totalcount + st->tpcount + st->done;
+  for (uint32_t i = 0; i < st->tpcount; ++i) s += lo[i] + hi[i];
+  return s;
+}
+
+extern "C" int clw_ckpt_write(const char* path, clw_work_status* st, const uint64_t* res_lo,
+                              const uint64_t* res_hi) {
+  static_assert(sizeof(clw_work_status) == 56, "workStatus layout");
+  if (!path || !st || (st->tpcount && (!res_lo || !res_hi))) return clw_fail(CLW_ERR_ARG, "clw_ckpt_write: null");
+  st->state_sum = ckpt_sum(st, res_lo, res_hi);
+  FILE* f = fopen(path, "wb");
+  if (!f) return clw_fail(CLW_ERR_IO, "cannot open %s", path);
+  bool ok = fwrite(st, sizeof(*st), 1, f) == 1;
+  for (uint32_t i = 0; ok && i < st->tpcount; ++i) {
+    const uint64_t pair[2] = {res_lo[i], res_hi[i]};
+    ok = fwrite(pair, 16, 1, f) == 1;
+  }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return clw_fail(CLW_ERR_IO, "cannot write checkpoint %s", path);
+  return CLW_OK;
+}
+
+namespace {
+// 0 invalid/missing, 1 valid, 2 valid and done.
+int ckpt_read_one(const char* path, uint64_t pmin, uint64_t pmax, uint32_t tpcount, clw_work_status* st,
+                  std::vector<uint64_t>& lo, std::vector<uint64_t>& hi) {
+  FILE* f = path ? fopen(path, "rb") : nullptr;
+  if (!f) return 0;
+  int r = 1;
+  if (fread(st, sizeof(*st), 1, f) != 1) r = 0;
+  if (r && (st->tpcount != tpcount || st->pmin != pmin || st->pmax != pmax)) r = 0;
+  if (r) {
+    lo.assign(tpcount, 0);
+    hi.assign(tpcount, 0);
+    for (uint32_t i = 0; r && i < tpcount; ++i) {
+      uint64_t pair[2];
+      if (fread(pair, 16, 1, f) != 1) r = 0;
+      lo[i] = pair[0];
+      hi[i] = pair[1];
+    }
+  }
+  fclose(f);
+  if (!r) return 0;
+  if (st->done) return 2;  // cl_wilson.cpp:192-194: a finished workunit
+  if (ckpt_sum(st, lo.data(), hi.data()) != st->state_sum) return 0;
+  return 1;
+}
+}  // namespace
+
+extern "C" int clw_ckpt_read(const char* path_a, const char* path_b, uint64_t pmin, uint64_t pmax, uint32_t tpcount,
+                             clw_work_status* st, uint64_t* res_lo, uint64_t* res_hi, int* which, int* result) {
+  if (!st || !result || (tpcount && (!res_lo || !res_hi))) return clw_fail(CLW_ERR_ARG, "clw_ckpt_read: null");
+  *result = 0;
+  if (which) *which = -1;
+  clw_work_status sa{}, sb{};
+  std::vector<uint64_t> la, ha, lb, hb;
+  const int ra = ckpt_read_one(path_a, pmin, pmax, tpcount, &sa, la, ha);
+  const int rb = ckpt_read_one(path_b, pmin, pmax, tpcount, &sb, lb, hb);
+  if (ra == 2 || rb == 2) {
+    *result = 2;
+    *st = ra == 2 ? sa : sb;
+    return CLW_OK;
+  }
+  int pick = -1;
+  if (ra == 1 && rb == 1) pick = sa.currp > sb.currp ? 0 : 1;  // cl_wilson.cpp:253-259
+  else if (ra == 1) pick = 0;
+  else if (rb == 1) pick = 1;
+  if (pick < 0) return CLW_OK;
+  *st = pick == 0 ? sa : sb;
+  const std::vector<uint64_t>& L = pick == 0 ? la : lb;
+  const std::vector<uint64_t>& H = pick == 0 ? ha : hb;
+  for (uint32_t i = 0; i < tpcount; ++i) {
+    res_lo[i] = L[i];
+    res_hi[i] = H[i];
+  }
+  if (which) *which = pick;
+  *result = 1;
+  return CLW_OK;
+}
+
+extern "C" int clw_session_set_residues(clw_session* S, const uint64_t* lo, const uint64_t* hi, uint64_t totalcount) {
+  if (!S || !lo || !hi) return clw_fail(CLW_ERR_ARG, "clw_session_set_residues: null argument");
+  const uint32_t n = (uint32_t)S->tp.size();
+  std::vector<uint64_t> a(n), b(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    const unsigned __int128 v = ((unsigned __int128)hi[k] << 64) | lo[k];
+    if (v >= (unsigned __int128)S->tp[k] * S->tp[k])
+      return clw_fail(CLW_ERR_ARG, "residue %u is not reduced mod p^2", k);
+    a[S->pos[k]] = lo[k];
+    b[S->pos[k]] = hi[k];
+  }
+  CK(clw_h2d(S->ctx, S->tab.acc_lo, a.data(), (size_t)n * 8, 0));
+  CK(clw_h2d(S->ctx, S->tab.acc_hi, b.data(), (size_t)n * 8, 1));
+  S->total_primes = totalcount;
+  return CLW_OK;
 }

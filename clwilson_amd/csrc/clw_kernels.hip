@@ -497,6 +497,81 @@ __global__ __launch_bounds__(256) void k_mul_records_lanes(CTX_PARAMS, const uin
   }
 }
 
+// ---------------------------------------------------------------------------
+// The reference's launch structure ("plain" path, one test prime per launch):
+// mullarge.cl:15-79 (grid-stride over the segment's primes, each converted to
+// Montgomery form and raised to its power by left-to-right binary powering,
+// a running product per work-item, a 256-lane LDS tree, one total per
+// work-group) and reduce.cl:11-44 (one work-group folds the group totals into
+// the residue). Same arithmetic as the rest of this file.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_plain_mul(CTX_PARAMS, const uint32_t* __restrict__ rec,
+                                                   const uint32_t* __restrict__ nrec, uint32_t tp, uint32_t nblk,
+                                                   uint32_t type, uint64_t* __restrict__ grp_lo,
+                                                   uint64_t* __restrict__ grp_hi) {
+  __shared__ Res<L> tot[256];
+  const uint32_t tid = threadIdx.x, G = gridDim.x;
+  const Ctx<L> c = LOAD_CTX(tp);
+  const uint32_t ew = 5 + type;
+  Res<L> A = c.one;  // Montgomery form of 1
+  const uint32_t b0 = (uint32_t)((uint64_t)nblk * blockIdx.x / G), b1 = (uint32_t)((uint64_t)nblk * (blockIdx.x + 1) / G);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t nr = nrec[b];
+    const uint32_t* rp = rec + (size_t)b * CLW_REC_CAP * CLW_REC_WORDS;
+    for (uint32_t r = tid; r < nr; r += 256) {
+      const uint4* w4 = reinterpret_cast<const uint4*>(rp + (size_t)r * CLW_REC_WORDS);
+      const uint4 lo4 = w4[0], hi4 = w4[1];
+      const uint32_t w[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+      const uint32_t e = w[ew];
+      if (e == 0) continue;
+      Res<L> q;
+#pragma unroll
+      for (int l = 0; l < L; ++l) q.v[l] = w[l];
+      const Res<L> base = mul_plain<L>(c.r2, q, c.n, c.ninv);  // Q R
+      A = mul_full<L>(A, mont_pow<L>(c, base, e), c.n, c.ninv);
+    }
+  }
+  tot[tid] = A;
+  __syncthreads();
+  for (uint32_t sz = 128; sz > 0; sz >>= 1) {
+    if (tid < sz) tot[tid] = mul_full<L>(tot[tid], tot[tid + sz], c.n, c.ninv);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const u128 v = res_to_u128<L>(tot[0]);
+    grp_lo[blockIdx.x] = (uint64_t)v;
+    grp_hi[blockIdx.x] = (uint64_t)(v >> 64);
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_plain_reduce(CTX_PARAMS, uint64_t* __restrict__ acc_lo,
+                                                      uint64_t* __restrict__ acc_hi,
+                                                      const uint64_t* __restrict__ grp_lo,
+                                                      const uint64_t* __restrict__ grp_hi, uint32_t tp,
+                                                      uint32_t ngroups) {
+  __shared__ Res<L> tot[256];
+  const uint32_t tid = threadIdx.x;
+  const Ctx<L> c = LOAD_CTX(tp);
+  Res<L> T = c.one;
+  for (uint32_t g = tid; g < ngroups; g += 256)
+    T = mul_full<L>(T, res_from_u128<L>(((u128)grp_hi[g] << 64) | grp_lo[g]), c.n, c.ninv);
+  tot[tid] = T;
+  __syncthreads();
+  for (uint32_t sz = 128; sz > 0; sz >>= 1) {
+    if (tid < sz) tot[tid] = mul_full<L>(tot[tid], tot[tid + sz], c.n, c.ninv);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    // canonical acc times the Montgomery-form total: acc * total exactly
+    const Res<L> A = res_from_u128<L>(((u128)acc_hi[tp] << 64) | acc_lo[tp]);
+    const u128 v = res_to_u128<L>(cond_sub<L>(mul_full<L>(A, tot[0], c.n, c.ninv), c.n));
+    acc_lo[tp] = (uint64_t)v;
+    acc_hi[tp] = (uint64_t)(v >> 64);
+  }
+}
+
 // Side list: {q, e} pairs with arbitrary 64-bit exponents (Montgomery powers).
 template <int L>
 __global__ __launch_bounds__(256) void k_mul_side(CTX_PARAMS, const uint64_t* __restrict__ side,
@@ -850,4 +925,22 @@ extern "C" int clw_finalize(clw_ctx* ctx, const clw_tp_table* t, uint64_t* quot,
   DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_finalize<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), t->p,
                                           t->type, t->acc_lo, t->acc_hi, t->count, quot, rem, acu));
   return clw_check_launch("finalize");
+}
+
+extern "C" int clw_plain_mul(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
+                             uint32_t tp, uint32_t nblk, uint32_t groups, uint64_t* grp_lo, uint64_t* grp_hi) {
+  if (!ctx || !t || !s || type > 2 || tp >= t->count || nblk > s->max_blocks || groups == 0)
+    return clw_fail(CLW_ERR_ARG, "clw_plain_mul: bad arguments");
+  const dim3 grid(groups), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_plain_mul<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), s->rec,
+                                          s->nrec, tp, nblk, type, grp_lo, grp_hi));
+  return clw_check_launch("plain_mul");
+}
+
+extern "C" int clw_plain_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp, uint32_t groups,
+                                const uint64_t* grp_lo, const uint64_t* grp_hi) {
+  if (!ctx || !t || tp >= t->count || groups == 0) return clw_fail(CLW_ERR_ARG, "clw_plain_reduce: bad arguments");
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_plain_reduce<LL>, dim3(1), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
+                                          t->acc_lo, t->acc_hi, grp_lo, grp_hi, tp, groups));
+  return clw_check_launch("plain_reduce");
 }

@@ -12,6 +12,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,8 @@
 static const uint64_t kMaxP = 0xFFFFFFFFFFFFFFFFull / 4;  // cl_wilson.h:9
 static const char* kResultFile = "results.txt";
 static const char* kGoodFile = "goodWilsonResults.txt";
+static const char* kStateA = "stateA.ckp";  // cl_wilson.h:5-6
+static const char* kStateB = "stateB.ckp";
 
 static void help() {
   printf("clwilson: search for Wilson primes (p-1)! = -1 (mod p^2) on an AMD Instinct GPU\n");
@@ -36,7 +39,7 @@ static void help() {
 
 struct Args {
   uint64_t pmin = 0, pmax = 0;
-  bool test = false, result_test = false;
+  bool test = false, result_test = false;  // -s, -r
   int device = 0;
 };
 
@@ -83,7 +86,75 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   q.resize(n);
   r.resize(n);
   clw_options opt = {0, 0, quiet ? 0u : 1u, 0, 0};
-  const int rc = clw_wilson_range(ctx, a.pmin, a.pmax, &opt, st, p.data(), q.data(), r.data(), n);
+  clw_session* S = nullptr;
+  if (clw_session_create(ctx, a.pmin, a.pmax, &opt, &S) != CLW_OK) {
+    printf("error: %s\n", clw_last_error());
+    return 1;
+  }
+  clw_range_stats info;
+  clw_session_info(S, &info);
+  // read_state (cl_wilson.cpp:1736-1777): resume from the newest valid file.
+  uint64_t currp = 2;
+  bool write_a_next = true;
+  time_t trickle = time(nullptr);
+  if (!a.test) {
+    std::vector<uint64_t> lo(n), hi(n);
+    clw_work_status ws;
+    int which = -1, res = 0;
+    if (clw_ckpt_read(kStateA, kStateB, a.pmin, a.pmax, (uint32_t)n, &ws, lo.data(), hi.data(), &which, &res) !=
+        CLW_OK) {
+      printf("error: %s\n", clw_last_error());
+      clw_session_destroy(S);
+      return 1;
+    }
+    if (res == 2) {
+      printf("Workunit complete.\n");
+      clw_session_destroy(S);
+      return 0;
+    }
+    if (res == 1) {
+      if (clw_session_set_residues(S, lo.data(), hi.data(), ws.totalcount) != CLW_OK) {
+        printf("error: %s\n", clw_last_error());
+        clw_session_destroy(S);
+        return 1;
+      }
+      currp = ws.currp;
+      trickle = (time_t)ws.trickle;
+      write_a_next = which == 1;
+      printf("Resuming search from checkpoint. Current P: %" PRIu64 "\n", currp);
+    }
+  }
+  auto checkpoint = [&](uint64_t cp, uint32_t done) -> int {
+    std::vector<uint64_t> lo(n), hi(n);
+    if (clw_session_residues(S, lo.data(), hi.data(), n) != CLW_OK) return 1;
+    clw_range_stats cur;
+    clw_session_info(S, &cur);
+    clw_work_status ws = {a.pmin, a.pmax, cp, (uint64_t)trickle, 0, cur.totalcount, (uint32_t)n, done};
+    if (clw_ckpt_write(write_a_next ? kStateA : kStateB, &ws, lo.data(), hi.data()) == CLW_OK)
+      write_a_next = !write_a_next;
+    else
+      fprintf(stderr, "Cannot write checkpoint to file. Continuing...\n");
+    return 0;
+  };
+  // Segment loop in windows, a checkpoint at least every 60 s
+  // (cl_wilson.cpp:1856-1929).
+  time_t last_ckpt = time(nullptr);
+  const uint64_t window = 8ull * 268492800ull;
+  int rc = CLW_OK;
+  for (uint64_t qlo = currp; rc == CLW_OK && qlo <= info.maxtarget;) {
+    const uint64_t qhi = std::min<uint64_t>(qlo + window, info.maxtarget + 1);
+    rc = clw_session_run(S, qlo, qhi);
+    qlo = qhi;
+    if (rc == CLW_OK && !a.test && time(nullptr) - last_ckpt > 60) {
+      last_ckpt = time(nullptr);
+      checkpoint(qlo, 0);
+      if (!quiet) printf("Checkpoint, Current P: %" PRIu64 "\n", qlo);
+    }
+  }
+  if (rc == CLW_OK) rc = clw_session_iterate(S);
+  if (rc == CLW_OK) rc = clw_session_finalize(S, st, p.data(), q.data(), r.data(), n);
+  if (rc == CLW_OK && !a.test) checkpoint(info.maxtarget + 1, 1);
+  clw_session_destroy(S);
   if (rc != CLW_OK) {
     fprintf(stderr, "error: %s\n", clw_last_error());
     printf("error: %s\n", clw_last_error());
@@ -200,6 +271,7 @@ static int self_test(clw_ctx* ctx) {
     Args a;
     a.pmin = kKats[k].pmin;
     a.pmax = kKats[k].pmax;
+    a.test = true;
     clw_range_stats st;
     const int rc = run_range(ctx, a, &st, true);
     const bool ok = rc == 0 && st.resultcount == kKats[k].results && st.checksum == kKats[k].checksum &&

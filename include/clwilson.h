@@ -183,6 +183,15 @@ int clw_mul_records_lanes(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buf
 int clw_mul_side(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
                  uint32_t tp_off, uint32_t tp_n, uint32_t K,
                  uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride);
+/* The reference's launch structure for one test prime ("plain" path):
+ * mullarge.cl (groups work-groups over the segment's records, per-record
+ * Montgomery conversion and binary powering, LDS tree, one total per group
+ * into grp_lo/hi) and reduce.cl (one work-group folds the totals into the
+ * residue of test prime tp). */
+int clw_plain_mul(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type, uint32_t tp,
+                  uint32_t nblk, uint32_t groups, uint64_t* grp_lo, uint64_t* grp_hi);
+int clw_plain_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp, uint32_t groups, const uint64_t* grp_lo,
+                     const uint64_t* grp_hi);
 /* reduce.cl: acc[tp_off+i] *= prod_{c<nparts} part[c * part_stride + i]. */
 int clw_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp_off, uint32_t tp_n, uint32_t nparts,
                const uint64_t* part_lo, const uint64_t* part_hi, uint32_t part_stride);
@@ -205,7 +214,9 @@ typedef struct {
   uint32_t verbose;        /* progress lines on stderr                        */
   uint32_t mode;           /* product kernel: 0 auto, 1 lane per test prime
                               (clw_mul_records), 2 wave per test prime
-                              (clw_mul_records_lanes)                         */
+                              (clw_mul_records_lanes), 3 the reference's
+                              launch structure, one test prime per launch
+                              (clw_plain_mul + clw_plain_reduce)              */
   uint32_t per_lane;       /* test primes per lane in clw_mul_records (0 auto) */
 } clw_options;
 
@@ -262,6 +273,32 @@ typedef struct {
 } clw_kernel_stats;
 int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
 int clw_session_reset_stats(clw_session* s);
+
+/* ---- checkpoint / resume (cl_wilson.cpp:110-319, 1734-1777) ------------- */
+
+/* The reference's checkpoint header (workStatus, cl_wilson.h:22-25), 56 bytes,
+ * followed in the file by tpcount (lo, hi) residue pairs. This build stores
+ * each residue in canonical form (value mod p^2, increasing p) where the
+ * reference stores Montgomery digits. state_sum = pmin + pmax + currp +
+ * trickle + totalcount + tpcount + done + sum of all residue words (mod 2^64). */
+typedef struct {
+  uint64_t pmin, pmax, currp, trickle, state_sum, totalcount;
+  uint32_t tpcount, done;
+} clw_work_status;
+
+/* write_state (cl_wilson.cpp:110-156): writes header + residues to path,
+ * filling st->state_sum. */
+int clw_ckpt_write(const char* path, clw_work_status* st, const uint64_t* res_lo, const uint64_t* res_hi);
+/* read_state (cl_wilson.cpp:159-288): reads both files, rejects a file that
+ * cannot be parsed, belongs to another range/count, or fails its checksum,
+ * and picks the one with the larger currp. *result = 0 (no valid file),
+ * 1 (resume: st and residues filled, *which = 0 for A / 1 for B),
+ * 2 (a valid file says done). res_* need room for tpcount entries. */
+int clw_ckpt_read(const char* path_a, const char* path_b, uint64_t pmin, uint64_t pmax, uint32_t tpcount,
+                  clw_work_status* st, uint64_t* res_lo, uint64_t* res_hi, int* which, int* result);
+/* Uploads canonical residues (increasing p) and the applied-prime count of a
+ * checkpoint into a session (the sclWrite of residues + setup(resume=1)). */
+int clw_session_set_residues(clw_session* s, const uint64_t* lo, const uint64_t* hi, uint64_t totalcount);
 
 /* Result line of cl_wilson.cpp:657-688 for one test prime into buf; returns
  * 1 if the prime is reported (quot = 0 or min(quot, p-quot) < 1000), else 0. */

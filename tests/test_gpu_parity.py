@@ -33,9 +33,10 @@ def test_selftest_case5_small_iterations(dev, golden):
     assert res.results[-1] == "563 is a Wilson prime\n"
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_small_ranges_bit_exact(dev, golden, mode):
-    """Every prime's Wilson quotient equals the oracle's, both product kernels."""
+    """Every prime's Wilson quotient equals the oracle's, for the lane-per-prime
+    and wave-per-prime kernels and the reference's one-prime-per-launch path."""
     for case in golden("oracle_small.json"):
         say("range", case["pmin"], case["pmax"], "mode", mode)
         res = dev.wilson_range(case["pmin"], case["pmax"], mode=mode)
@@ -148,9 +149,10 @@ def test_cgh_values_large(dev, golden):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("k", [0, 1])
-def test_selftest_cases_1_2(dev, golden, k):
-    """cl_wilson.cpp:2007-2038: p ~ 1.1e12-1.2e12 (type 0 and type 1)."""
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 6])
+def test_selftest_large_primes(dev, golden, k):
+    """cl_wilson.cpp:2007-2072, 2110-2123: single primes 1.1e12-1.75e13 (all
+    three types; case 7 is in config 3's magnitude band)."""
     kat = golden("kat_selftest.json")[k]
     res = dev.wilson_range(kat["pmin"], kat["pmax"])
     say("kat", k + 1, round(res.seconds, 2))

@@ -1,0 +1,53 @@
+"""Checkpoint / resume on the GPU and the command line end to end."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import clwilson_amd as cw
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_resume_equals_uninterrupted(dev, tmp_path):
+    lo, hi = 10**9, 10**9 + 20000
+    ref = dev.wilson_range(lo, hi)
+    a, b = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    cut = 123_456_789
+    with cw.Session(dev, lo, hi) as s:
+        s.run(2, cut)
+        s.checkpoint(a, cut)
+        n = s.n
+    res, which, st, vals = cw.ckpt_read(a, b, lo, hi, n)
+    assert res == 1 and which == 0 and st.currp == cut
+    with cw.Session(dev, lo, hi) as s:
+        s.set_residues(vals, st.totalcount)
+        s.run(st.currp, s.maxtarget + 1)
+        s.iterate()
+        out = s.finalize()
+    assert np.array_equal(out.quot, ref.quot) and out.checksum == ref.checksum
+    assert out.totalcount == ref.totalcount
+
+
+def test_cli_range_results_file_and_done_state(tmp_path):
+    exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
+    r = subprocess.run([exe, "-p", "200", "-P", "564"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(tmp_path / "results.txt").read().splitlines()
+    assert len(lines) == 58 and lines[-1] == "00000000000080A3"
+    assert lines[-2] == "563 is a Wilson prime"
+    assert os.path.exists(tmp_path / "stateA.ckp")
+    r = subprocess.run([exe, "-p", "200", "-P", "564"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "Workunit complete." in r.stdout
+
+
+def test_cli_suffix_arguments(tmp_path):
+    exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
+    r = subprocess.run([exe, "-p", "86M", "-P", "86010000"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    import oracle as O
+    arr, st = O.wilson_range(86000000, 86010000)
+    assert open(tmp_path / "results.txt").read().splitlines()[-1] == f"{st['checksum']:016X}"
