@@ -79,7 +79,7 @@ class WorkStatus(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
-                ("mode", C.c_uint32), ("per_lane", C.c_uint32)]
+                ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -152,6 +152,18 @@ def lib():
         "clw_session_set_residues": (i32, [vp, vp, vp, u64]),
         "clw_plain_mul": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, vp, vp]),
         "clw_plain_reduce": (i32, [vp, P(TpTable), u32, u32, vp, vp]),
+        "clw_iterate_from": (i32, [vp, P(TpTable), vp]),
+        "clw_batch_create": (i32, [vp, vp, vp, u32, P(Options), P(vp)]),
+        "clw_batch_destroy": (i32, [vp]),
+        "clw_batch_size": (u32, [vp]),
+        "clw_batch_info": (i32, [vp, u32, P(RangeStats)]),
+        "clw_batch_run": (i32, [vp, u64, u64]),
+        "clw_batch_applied": (u64, [vp]),
+        "clw_batch_residues": (i32, [vp, u32, vp, vp, u64]),
+        "clw_batch_iterate": (i32, [vp]),
+        "clw_batch_finalize": (i32, [vp]),
+        "clw_batch_results": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
+        "clw_batch_kernel_stats": (i32, [vp, P(KernelStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -365,6 +377,94 @@ class Session:
             if line:
                 res.results.append(line)
         return res
+
+
+class Batch:
+    """Many ranges searched together over one shared sieve (clw_batch_*).
+    Each range's results equal clw_wilson_range's on it, bit for bit."""
+
+    def __init__(self, dev: "Device", ranges: list[tuple[int, int]], segment_span: int = 0,
+                 target_waves: int = 0, per_lane: int = 0, graph: bool = True):
+        if not ranges:
+            raise ValueError("a batch needs at least one range")
+        self.dev = dev
+        self.h = C.c_void_p()
+        self.ranges = [(int(a), int(b)) for a, b in ranges]
+        lo = np.array([a for a, _ in self.ranges], dtype=np.uint64)
+        hi = np.array([b for _, b in self.ranges], dtype=np.uint64)
+        opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0)
+        check(lib().clw_batch_create(dev.ctx, lo.ctypes.data, hi.ctypes.data, len(self.ranges), C.byref(opt),
+                                     C.byref(self.h)), "batch_create")
+        self.info = []
+        for r in range(len(self.ranges)):
+            st = RangeStats()
+            check(lib().clw_batch_info(self.h, r, C.byref(st)))
+            self.info.append(st)
+        self.maxtarget = max(st.maxtarget for st in self.info)
+
+    def close(self) -> None:
+        if self.h:
+            lib().clw_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self) -> int:
+        return len(self.ranges)
+
+    def run(self, qlo: int, qhi: int) -> None:
+        check(lib().clw_batch_run(self.h, qlo, qhi), f"batch_run [{qlo}, {qhi})")
+
+    def applied(self) -> int:
+        return lib().clw_batch_applied(self.h)
+
+    def residues(self, r: int) -> list[int]:
+        n = self.info[r].tpcount
+        lo = np.zeros(max(n, 1), dtype=np.uint64)
+        hi = np.zeros(max(n, 1), dtype=np.uint64)
+        check(lib().clw_batch_residues(self.h, r, lo.ctypes.data, hi.ctypes.data, n))
+        return [int(a) | (int(b) << 64) for a, b in zip(lo[:n], hi[:n])]
+
+    def iterate(self) -> None:
+        check(lib().clw_batch_iterate(self.h), "batch_iterate")
+
+    def finalize(self) -> None:
+        check(lib().clw_batch_finalize(self.h), "batch_finalize")
+
+    def results(self, r: int, allow_check_failure: bool = False) -> RangeResult:
+        n = self.info[r].tpcount
+        p = np.zeros(max(n, 1), dtype=np.uint64)
+        q = np.zeros(max(n, 1), dtype=np.uint64)
+        m = np.zeros(max(n, 1), dtype=np.uint64)
+        st = RangeStats()
+        rc = lib().clw_batch_results(self.h, r, C.byref(st), p.ctypes.data, q.ctypes.data, m.ctypes.data, n)
+        if rc != CLW_OK and not (allow_check_failure and rc == CLW_ERR_CHECK):
+            check(rc, f"batch_results {r}")
+        pmin, pmax = self.ranges[r]
+        res = RangeResult(pmin, pmax, p[:n], q[:n], m[:n], st.checksum, st.totalcount, list(st.typeTarget),
+                          list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs, st.seconds)
+        for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
+            line = format_result(pi, qi)
+            if line:
+                res.results.append(line)
+        return res
+
+    def search(self) -> list[RangeResult]:
+        """The whole plain method for every range: all sieving primes up to
+        the largest target, iterate, finalize."""
+        self.run(2, self.maxtarget + 1)
+        self.iterate()
+        self.finalize()
+        return [self.results(r) for r in range(len(self.ranges))]
+
+    def kernel_stats(self) -> dict:
+        ks = KernelStats()
+        check(lib().clw_batch_kernel_stats(self.h, C.byref(ks)))
+        return ks.as_dict()
 
 
 def ckpt_write(path: str, st: WorkStatus, values: list[int]) -> None:

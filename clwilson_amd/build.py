@@ -34,7 +34,7 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["clw_kernels.hip", "clw_runtime.hip", "cl_wilson.hip"]
+LIB_SOURCES = ["clw_kernels.hip", "clw_runtime.hip", "cl_wilson.hip", "clw_batch.hip"]
 EXE_SOURCES = ["main.cpp", "putil.cpp"]
 
 

@@ -94,23 +94,27 @@ __device__ __forceinline__ void side_push(uint64_t* __restrict__ side, uint32_t*
   }
 }
 
-// One block = CLW_BLOCK_SPAN consecutive integers from base + b*SPAN.
-__global__ __launch_bounds__(256) void k_sieve_pack(uint64_t base, uint64_t lo, uint64_t hi, uint64_t T0,
-                                                    uint64_t T1, uint64_t T2, uint32_t nbases, uint32_t side_cap,
-                                                    uint32_t* __restrict__ rec, uint32_t* __restrict__ nrec,
-                                                    uint32_t* __restrict__ nprimes, uint64_t* __restrict__ side0,
-                                                    uint64_t* __restrict__ side1, uint64_t* __restrict__ side2,
-                                                    uint32_t* __restrict__ side_count, uint32_t* __restrict__ flags,
-                                                    unsigned long long* __restrict__ total) {
-  __shared__ uint32_t bm[TURNS / 4];      // one byte per wheel turn, bit r <-> residue c_wheel[r]
-  __shared__ uint16_t cand[CAND_CAP];     // offsets of survivors, increasing
-  __shared__ uint16_t prm[PRIME_CAP];     // offsets of primes, increasing
-  __shared__ uint32_t pass[CAND_CAP / 32];
-  __shared__ uint32_t wsum[4];
-  __shared__ uint32_t s_fail;
+// LDS of one sieve block.
+struct SieveLds {
+  uint32_t bm[TURNS / 4];   // one byte per wheel turn, bit r <-> residue c_wheel[r]
+  uint16_t cand[CAND_CAP];  // offsets of survivors, increasing
+  uint16_t prm[PRIME_CAP];  // offsets of primes, increasing
+  uint32_t pass[CAND_CAP / 32];
+  uint32_t wsum[4];
+  uint32_t s_fail;
+};
 
-  const uint32_t tid = threadIdx.x, b = blockIdx.x;
-  const uint64_t blo = base + (uint64_t)b * CLW_BLOCK_SPAN;
+// The exact primes of [blo, blo + SPAN) clipped to [lo, hi), as increasing
+// offsets from blo in S.prm[0..return). Called by all 256 threads.
+__device__ uint32_t sieve_block(SieveLds& S, uint64_t blo, uint64_t lo, uint64_t hi, uint32_t nbases,
+                                uint32_t* __restrict__ flags) {
+  uint32_t* const bm = S.bm;
+  uint16_t* const cand = S.cand;
+  uint16_t* const prm = S.prm;
+  uint32_t* const pass = S.pass;
+  uint32_t* const wsum = S.wsum;
+  uint32_t& s_fail = S.s_fail;
+  const uint32_t tid = threadIdx.x;
   const uint64_t bhi = blo + CLW_BLOCK_SPAN;
   const uint64_t vlo = blo > lo ? blo : lo;
   const uint64_t vhi = bhi < hi ? bhi : hi;
@@ -212,6 +216,23 @@ __global__ __launch_bounds__(256) void k_sieve_pack(uint64_t base, uint64_t lo, 
       np = s_fail;
     }
   }
+
+  return np;
+}
+
+// One block = CLW_BLOCK_SPAN consecutive integers from base + b*SPAN.
+__global__ __launch_bounds__(256) void k_sieve_pack(uint64_t base, uint64_t lo, uint64_t hi, uint64_t T0,
+                                                    uint64_t T1, uint64_t T2, uint32_t nbases, uint32_t side_cap,
+                                                    uint32_t* __restrict__ rec, uint32_t* __restrict__ nrec,
+                                                    uint32_t* __restrict__ nprimes, uint64_t* __restrict__ side0,
+                                                    uint64_t* __restrict__ side1, uint64_t* __restrict__ side2,
+                                                    uint32_t* __restrict__ side_count, uint32_t* __restrict__ flags,
+                                                    unsigned long long* __restrict__ total) {
+  __shared__ SieveLds S;
+  const uint32_t tid = threadIdx.x, b = blockIdx.x;
+  const uint64_t blo = base + (uint64_t)b * CLW_BLOCK_SPAN;
+  const uint32_t np = sieve_block(S, blo, lo, hi, nbases, flags);
+  const uint16_t* const prm = S.prm;
 
   // Pair consecutive primes into records {Q limbs[5], e0, e1, e2}.
   uint32_t nr = (np + 1) / 2;
@@ -627,13 +648,14 @@ template <int L>
 __global__ __launch_bounds__(256) void k_iterate(CTX_PARAMS, const uint64_t* __restrict__ tp_target,
                                                  const uint32_t* __restrict__ type, uint64_t* __restrict__ acc_lo,
                                                  uint64_t* __restrict__ acc_hi, uint32_t count, uint64_t from0,
-                                                 uint64_t from1, uint64_t from2) {
+                                                 uint64_t from1, uint64_t from2,
+                                                 const uint64_t* __restrict__ from_tp) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= count) return;  // whole wave
   const Ctx<L> c = LOAD_CTX(i);
   const uint32_t t = type[i];
-  const uint64_t from = t == 0 ? from0 : (t == 1 ? from1 : from2);
+  const uint64_t from = from_tp ? from_tp[i] : (t == 0 ? from0 : (t == 1 ? from1 : from2));
   const uint64_t to = tp_target[i];
   const uint64_t len = to > from ? to - from : 0;
   uint64_t per = (len + 63) / 64;
@@ -787,6 +809,8 @@ __global__ __launch_bounds__(256) void k_finalize(CTX_PARAMS, const uint64_t* __
   }
 }
 
+#include "clw_batch_kernels.inc"
+
 // ---------------------------------------------------------------------------
 // Launchers (extern "C", see include/clwilson.h)
 // ---------------------------------------------------------------------------
@@ -914,7 +938,17 @@ extern "C" int clw_iterate(clw_ctx* ctx, const clw_tp_table* t, const uint64_t f
   if (t->count == 0) return CLW_OK;
   const dim3 grid(ceil_div(t->count, 4)), block(256);
   DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_iterate<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), t->tp_target,
-                                          t->type, t->acc_lo, t->acc_hi, t->count, from[0], from[1], from[2]));
+                                          t->type, t->acc_lo, t->acc_hi, t->count, from[0], from[1], from[2],
+                                          (const uint64_t*)nullptr));
+  return clw_check_launch("iterate");
+}
+
+extern "C" int clw_iterate_from(clw_ctx* ctx, const clw_tp_table* t, const uint64_t* from_tp) {
+  if (!ctx || !t || !from_tp) return clw_fail(CLW_ERR_ARG, "clw_iterate_from: null argument");
+  if (t->count == 0) return CLW_OK;
+  const dim3 grid(ceil_div(t->count, 4)), block(256);
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_iterate<LL>, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), t->tp_target,
+                                          t->type, t->acc_lo, t->acc_hi, t->count, 0ull, 0ull, 0ull, from_tp));
   return clw_check_launch("iterate");
 }
 
@@ -944,3 +978,59 @@ extern "C" int clw_plain_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp
                                           t->acc_lo, t->acc_hi, grp_lo, grp_hi, tp, groups));
   return clw_check_launch("plain_reduce");
 }
+
+// ---- batched engine launchers (internal: called by clw_batch.hip) ----------
+extern "C" int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uint32_t max_blocks,
+                                uint32_t* flags, uint64_t* total) {
+  hipLaunchKernelGGL(k_sieve_pairs, dim3(max_blocks), dim3(256), 0, clw_stream(ctx), (const BatchSeg*)seg, rec, nrec,
+                     max_blocks, flags, (unsigned long long*)total);
+  return clw_check_launch("sieve_pairs");
+}
+
+extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
+                              const uint32_t* item_gt, uint32_t items_per_chunk, uint32_t K, const uint32_t* rec,
+                              const uint32_t* nrec, void* seg, uint32_t qslot, uint32_t extra_block, uint32_t grid,
+                              uint64_t* part_lo, uint64_t* part_hi) {
+  // grid 0: as many resident blocks as the device holds (persistent kernel),
+  // at most one per work item.
+  const uint32_t n_items = items_per_chunk * K;
+  DISPATCH_LK(t->limbs, per_lane, {
+    uint32_t g = grid;
+    if (g == 0) {
+      int per_cu = 0;
+      const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_mul<LL, KK>, 256, 0);
+      if (e != hipSuccess) return clw_fail_hip(e, "occupancy query");
+      g = (uint32_t)(per_cu > 0 ? per_cu : 1) * (uint32_t)ctx->prop.multiProcessorCount;
+    }
+    if (g > n_items) g = n_items;
+    if (g == 0) return CLW_OK;
+    hipLaunchKernelGGL((k_batch_mul<LL, KK>), dim3(g), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
+                       (const BatchGroup*)groups, item_gt, items_per_chunk, K, rec, nrec, (BatchSeg*)seg, qslot,
+                       extra_block, part_lo, part_hi);
+  });
+  return clw_check_launch("batch_mul");
+}
+
+extern "C" int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
+                                 uint32_t K, const uint64_t* part_lo, const uint64_t* part_hi) {
+  if (t->count == 0) return CLW_OK;
+  DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_batch_reduce<LL>, dim3(ceil_div(t->count, 256)), dim3(256), 0,
+                                          clw_stream(ctx), CTX_ARGS(t), t->acc_lo, t->acc_hi, tp_group,
+                                          (const BatchGroup*)groups, t->count, K, part_lo, part_hi));
+  return clw_check_launch("batch_reduce");
+}
+
+extern "C" int clwi_seg_advance(clw_ctx* ctx, void* seg, uint32_t* flags, uint64_t* total_prev, uint64_t* total) {
+  hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(64), 0, clw_stream(ctx), (BatchSeg*)seg, flags,
+                     (unsigned long long*)total_prev, (unsigned long long*)total);
+  return clw_check_launch("seg_advance");
+}
+
+extern "C" int clwi_small_primes_block(clw_ctx* ctx, uint64_t qlo, uint64_t qhi, uint32_t* rec, uint32_t* nrec,
+                                       uint32_t blk) {
+  hipLaunchKernelGGL(k_small_primes_block, dim3(1), dim3(64), 0, clw_stream(ctx), qlo, qhi, rec, nrec, blk);
+  return clw_check_launch("small_primes_block");
+}
+
+extern "C" size_t clwi_batch_seg_size() { return sizeof(BatchSeg); }
+extern "C" size_t clwi_batch_group_size() { return sizeof(BatchGroup); }

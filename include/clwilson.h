@@ -199,6 +199,10 @@ int clw_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp_off, uint32_t tp
 /* iterate.cl: acc[i] *= prod_{n = from[type]+1}^{tp_target[i]} n for every
  * test prime (from = the type targets). */
 int clw_iterate(clw_ctx* ctx, const clw_tp_table* t, const uint64_t from[3]);
+/* The same with a start per test prime: acc[i] *= prod_{n = from_tp[i]+1}^
+ * {tp_target[i]} n (from_tp is a device array of count entries). Used when
+ * one table holds test primes of several ranges, each with its own targets. */
+int clw_iterate_from(clw_ctx* ctx, const clw_tp_table* t, const uint64_t* from_tp);
 
 /* find.cl + processResult (cl_wilson.cpp:542-870): from acc = T_p! mod p^2
  * compute (p-1)! + 1 = rem + p*quot (mod p^2) with the type identities; the
@@ -218,6 +222,8 @@ typedef struct {
                               launch structure, one test prime per launch
                               (clw_plain_mul + clw_plain_reduce)              */
   uint32_t per_lane;       /* test primes per lane in clw_mul_records (0 auto) */
+  uint32_t graph;          /* batches: replay the segment step as a captured
+                              HIP graph (0 = launch it eagerly)               */
 } clw_options;
 
 typedef struct {
@@ -273,6 +279,39 @@ typedef struct {
 } clw_kernel_stats;
 int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
 int clw_session_reset_stats(clw_session* s);
+
+/* ---- batches: many ranges over one shared sieve ----------------------- */
+
+/* The reference searches one range per process (main.cpp:313-405 ->
+ * cl_wilson()); a batch runs the same plain method for up to thousands of
+ * ranges at once. Each segment of sieving primes is sieved once and applied
+ * to every range whose targets reach it: a persistent product kernel per limb
+ * band takes (chunk, range-type group, tile) items from a device work queue
+ * and computes each group's Legendre exponents on the fly. Results are those
+ * of clw_wilson_range on each range, bit for bit. Ranges must have p below
+ * 8.2e17 (at most 4 limbs); wider p are served by sessions. */
+typedef struct clw_batch clw_batch;
+int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64_t* pmax, uint32_t nranges,
+                     const clw_options* opt, clw_batch** out);
+int clw_batch_destroy(clw_batch* b);
+uint32_t clw_batch_size(const clw_batch* b);
+/* Counts and targets of range r (as clw_session_info). */
+int clw_batch_info(const clw_batch* b, uint32_t r, clw_range_stats* stats);
+/* Applies the sieving primes q in [qlo, qhi), clipped to [2, the largest
+ * target of the batch], to every range (disjoint windows across calls). */
+int clw_batch_run(clw_batch* b, uint64_t qlo, uint64_t qhi);
+/* Sieving primes applied so far (all windows). */
+uint64_t clw_batch_applied(const clw_batch* b);
+/* Canonical running residues of range r, in increasing p. */
+int clw_batch_residues(clw_batch* b, uint32_t r, uint64_t* lo, uint64_t* hi, uint64_t cap);
+int clw_batch_iterate(clw_batch* b);
+int clw_batch_finalize(clw_batch* b);
+/* Range r after clw_batch_finalize: stats (checksum, results, last result;
+ * totalcount is not tracked per range and reads 0) and per-prime outputs. */
+int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, uint64_t* out_p,
+                      uint64_t* out_quot, uint64_t* out_rem, uint64_t out_cap);
+/* segments, sieve_integers and mul_ms (wall time of clw_batch_run). */
+int clw_batch_kernel_stats(const clw_batch* b, clw_kernel_stats* out);
 
 /* ---- checkpoint / resume (cl_wilson.cpp:110-319, 1734-1777) ------------- */
 

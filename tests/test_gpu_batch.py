@@ -1,0 +1,135 @@
+"""GPU parity of the batched engine: many ranges over one shared sieve.
+
+Every range of a batch must give exactly what a single-range run gives (the
+session path, itself pinned to the oracle and the reference's self-test
+answers), whatever the other ranges in the batch, the window splits, the
+segment span, the graph replay or the work-queue chunking. Integer work, so
+every comparison is bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import clwilson_amd as cw
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _same(a, b):
+    assert (a.pmin, a.pmax) == (b.pmin, b.pmax)
+    assert np.array_equal(a.p, b.p)
+    assert np.array_equal(a.quot, b.quot)
+    assert np.array_equal(a.rem, b.rem)
+    assert a.checksum == b.checksum
+    assert (a.resultcount, a.lastp, a.lastw) == (b.resultcount, b.lastp, b.lastw)
+    assert a.results == b.results
+    assert a.type_targets == b.type_targets and a.tpcnt == b.tpcnt
+
+
+# Ranges of every magnitude up to ~1e9 (targets up to 5e8: full searches
+# take a second): the 2-limb band and, above 759,250,125, the 3-limb band.
+MIX = [(5, 20000), (100000, 120000), (1000000, 1020000), (1999000, 2000000), (86000000, 86010000),
+       (759245000, 759255000), (1000000000, 1000004000), (999999000, 1000000000), (563, 564)]
+
+
+def test_batch_matches_oracle_fixtures(dev, golden):
+    """A batch of the oracle's small ranges returns the oracle's quotients."""
+    cases = golden("oracle_small.json")
+    say("batch of", len(cases), "oracle ranges")
+    with cw.Batch(dev, [(c["pmin"], c["pmax"]) for c in cases]) as b:
+        res = b.search()
+    for c, r in zip(cases, res):
+        assert [int(x) for x in r.p] == c["p"]
+        assert [int(x) for x in r.quot] == c["quot"]
+        assert not r.rem.any()
+        assert r.checksum_hex == c["checksum"]
+
+
+def test_batch_selftest_case5(dev, golden):
+    """cl_wilson.cpp:2076-2089 inside a batch: [200, 564), Wilson prime 563."""
+    kat = golden("kat_selftest.json")[4]
+    with cw.Batch(dev, [(kat["pmin"], kat["pmax"]), (100000, 100100)]) as b:
+        r = b.search()[0]
+    assert r.resultcount == kat["results"]
+    assert r.checksum_hex == kat["checksum"]
+    assert (r.lastp, r.lastw) == (kat["lastp"], kat["lastw"])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_single_ranges(dev, graph):
+    """Mixed magnitudes and both limb bands: each range equals its own run."""
+    say("mixed batch, graph", graph)
+    with cw.Batch(dev, MIX, graph=graph) as b:
+        got = b.search()
+    for (lo, hi), r in zip(MIX, got):
+        say("  range", lo, hi)
+        _same(r, dev.wilson_range(lo, hi))
+
+
+def test_batch_alone_vs_mix(dev):
+    """A range's results do not depend on the batch around it."""
+    with cw.Batch(dev, MIX) as b:
+        mixed = b.search()
+    for k in (0, 4, 6):
+        with cw.Batch(dev, [MIX[k]]) as b:
+            _same(b.search()[0], mixed[k])
+
+
+def test_batch_windows_and_span_invariance(dev):
+    """Window splits (including ones that cut 2, 3, 5 off) and a short
+    segment span (many graph replays per window) change nothing."""
+    ranges = MIX[2:7]
+    with cw.Batch(dev, ranges) as b:
+        ref = b.search()
+    cuts = [2, 4, 6, 1000, 65537, 12345678, 1 << 62]
+    say("windows", cuts)
+    with cw.Batch(dev, ranges, segment_span=3 * 61440) as b:
+        for a, z in zip(cuts, cuts[1:]):
+            b.run(a, z)
+        b.iterate()
+        b.finalize()
+        got = [b.results(r) for r in range(len(ranges))]
+    for x, y in zip(ref, got):
+        _same(x, y)
+
+
+def test_batch_many_small_ranges(dev):
+    """256 narrow ranges (768 groups, several chunks per group)."""
+    rng = np.random.default_rng(7)
+    starts = sorted(int(x) for x in rng.integers(10**6, 6 * 10**8, size=256))
+    ranges = [(s, s + int(w)) for s, w in zip(starts, rng.integers(200, 3000, size=256))]
+    say("256 ranges")
+    with cw.Batch(dev, ranges) as b:
+        got = b.search()
+    for k in (0, 1, 100, 200, 255):
+        _same(got[k], dev.wilson_range(*ranges[k]))
+
+
+@pytest.mark.parametrize("window", [(2, 300000000), (150000000, 2000000000)])
+def test_batch_window_residues_all_bands(dev, window):
+    """Partial products over a window of sieving primes for ranges in the 2,
+    3 and 4-limb bands equal the session's over the same window."""
+    ranges = [(600000000, 600002000), (3 * 10**12, 3 * 10**12 + 2000), (10**14, 10**14 + 3000)]
+    qlo, qhi = window
+    with cw.Batch(dev, ranges) as b:
+        assert [st.limbs for st in b.info] == [2, 3, 4]
+        b.run(qlo, qhi)
+        for r, (lo, hi) in enumerate(ranges):
+            say("window", window, "range", lo, hi)
+            with cw.Session(dev, lo, hi) as s:
+                s.run(qlo, qhi)
+                assert b.residues(r) == s.residues()
+
+
+def test_batch_rejects_bad_ranges(dev):
+    with pytest.raises(cw.ClwError):
+        cw.Batch(dev, [(10, 10)])
+    with pytest.raises(cw.ClwError):
+        cw.Batch(dev, [(10**18, 10**18 + 100)])  # 5-limb band: sessions only
+    with pytest.raises(cw.ClwError):
+        cw.Batch(dev, [(100, 100 + 10**7 + 1)])
