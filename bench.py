@@ -1,6 +1,7 @@
 """Benchmark: primes tested/s at p ~ 2e13 (every prime in [2e13, 2e13+1e7)).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--engine batch|session] [--batch-steps S]
 
 A full search of this range applies ~6.24e16 sieving-prime factors (days of
 GPU time), so the run is measured in windows and extrapolated in closed form:
@@ -8,22 +9,30 @@ GPU time), so the run is measured in windows and extrapolated in closed form:
 * The sieving-prime axis [2, 1e13] has three regions by the types whose
   factorial target is still above q: all three types (q <= T0 = 3.33e12),
   types 1-2 (q <= T1 = 5e12) and type 2 only (q <= T2 = 1e13).
-* A step = one segment (clw_session_run over SPAN integers) of that axis for
-  all 327,107 test primes: sieve + products + side powers + reduction.
-  Steps cycle through windows just below T0, T1 and T2.
+* A step = one segment (SPAN integers) of that axis for all 327,107 test
+  primes: sieve + products + reduction. Steps cycle through windows just
+  below T0, T1 and T2.
 * Each region's rate (factor applications/s, measured over its steps) times
   its factor count (test primes of the active types x sieving primes of the
   region, pi from the logarithmic integral) gives its time; iterate and the
   final check are timed once on top. primes/s = 327,107 / total time.
 
+The same windows are timed on the reference's launch structure (one test
+prime per launch pair, the "plain" path) for the batched/plain ratio, and
+the 256-range batched workload (starts log-uniform in [1e9, 1e14] from seed
+1234, widths uniform in [1e5, 1e7]) is measured the same way at the
+quantiles of its factor-application mass.
+
 The roofline line is for the dominant kernel (the record products): the
-algorithmic 30x30-bit partial products per launch (21 per record per test
-prime with 3 limbs) over the mean launch time from HIP events, against
-gfx950's peak v_mad_u64_u32 issue rate (CUs x 64 lanes x clock).
+algorithmic 30x30-bit partial products per launch (2L^2 + L per record per
+test prime with L limbs) over the mean launch time from HIP events on the
+library's stream, against gfx950's peak v_mad_u64_u32 issue rate (CUs x 64
+lanes x clock).
 """
 from __future__ import annotations
 
 import argparse
+import bisect
 import json
 import math
 import os
@@ -33,33 +42,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from clwilson_amd.workloads import batch_ranges, pi_approx  # noqa: E402
+
 METRIC = "primes tested/s at p≈2e13 (batched vs plain), % of int-MAD roofline"
 PMIN, PMAX = 20_000_000_000_000, 20_000_010_000_000
 SPAN = 4370 * 61440  # integers per step (one default segment)
 
 
-def li(x: float) -> float:
-    """Logarithmic integral Li(x) = li(x) - li(2) by Ramanujan's series."""
-    g = 0.5772156649015329
-    lx = math.log(x)
-    s, term, k = 0.0, 1.0, 1
-    inner = 0.0
-    while True:
-        term *= lx / k
-        if (k - 1) % 2 == 0:
-            inner += 1.0 / (2 * ((k - 1) // 2) + 1)
-        add = term * ((-1) ** (k - 1)) / (2 ** (k - 1)) * inner
-        s += add
-        if abs(add) < 1e-17 * abs(s) and k > 10:
-            break
-        k += 1
-    return g + math.log(lx) + math.sqrt(x) * s - 1.045163780117492784
-
-
-def pi_approx(x: float) -> float:
-    """pi(x) ~ Li(x) - Li(sqrt x)/2 (Riemann's leading terms); relative error
-    ~1e-7 at 1e12-1e13, far below the timing noise."""
-    return li(x) - 0.5 * li(math.sqrt(x))
+def say(*a):
+    print("#", *a, file=sys.stderr, flush=True)
 
 
 def regions(T, cnt):
@@ -72,6 +63,71 @@ def regions(T, cnt):
         nprimes = pi_approx(b - 1) - (pi_approx(a - 1) if a > 2 else 0.0)
         out.append((name, a, b, types, nprimes * sum(cnt[t] for t in types)))
     return out
+
+
+class SessionRunner:
+    """One range through clw_session_* (lane or wave per test prime)."""
+
+    def __init__(self, cw, dev, mode, per_lane):
+        self.cw, self.dev = cw, dev
+        self.s = cw.Session(dev, PMIN, PMAX, mode=mode, per_lane=per_lane, segment_span=SPAN)
+        self.T, self.cnt, self.limbs = self.s.type_targets, self.s.tpcnt, self.s.limbs
+        self.kp = per_lane or int(cw.lib().clw_default_per_lane(self.limbs))
+        lane = mode in (0, 1)
+        self.kernel = f"k_mul_records<{self.limbs},{self.kp}>" if lane else f"k_mul_records_lanes<{self.limbs}>"
+        self.name = "session, lane per test prime" if lane else "session, wave per test prime"
+
+    def run(self, lo, hi):
+        self.s.run(lo, hi)
+
+    def stats(self):
+        return self.s.kernel_stats()
+
+    def reset(self):
+        self.s.reset_stats()
+
+    def tail(self):
+        self.s.iterate()
+        self.s.finalize(allow_check_failure=True)  # windows only: rem != 0 expected
+
+    def close(self):
+        self.s.close()
+
+
+class BatchRunner:
+    """The range as a one-range batch (clw_batch_*), steps launched eagerly so
+    HIP events time each kernel."""
+
+    def __init__(self, cw, dev, per_lane, tune):
+        self.cw, self.dev = cw, dev
+        self.b = cw.Batch(dev, [(PMIN, PMAX)], per_lane=per_lane, segment_span=SPAN, graph=False)
+        st = self.b.info[0]
+        self.T, self.cnt, self.limbs = list(st.typeTarget), list(st.tpcnt), st.limbs
+        self.tune = self.b.autotune(st.typeTarget[0]) if tune else None
+        self.kp = self.tune[0]["per_lane"] if tune else (per_lane or int(cw.lib().clw_default_per_lane(self.limbs)))
+        self.kernel = f"k_batch_mul<{self.limbs},{self.kp}>"
+        self.name = "batched engine (one-range batch, persistent work-queue kernel)"
+        self._base = None
+
+    def run(self, lo, hi):
+        self.b.run(lo, hi)
+
+    def stats(self):
+        ks = self.b.kernel_stats()
+        if self._base:
+            ks = {k: ks[k] - self._base[k] for k in ks}
+        return ks
+
+    def reset(self):
+        self._base = None
+        self._base = self.b.kernel_stats()
+
+    def tail(self):
+        self.b.iterate()
+        self.b.finalize()
+
+    def close(self):
+        self.b.close()
 
 
 def cpu_baseline(T, cnt, tp, seconds_budget: float) -> dict:
@@ -119,7 +175,6 @@ def cpu_baseline(T, cnt, tp, seconds_budget: float) -> dict:
 def measure_plain(dev, cw, regs, nxt, steps):
     """The reference's launch structure on the same workload: one test prime
     per launch pair (mullarge + reduce), one window per region."""
-    out = {}
     with cw.Session(dev, PMIN, PMAX, mode=3) as s:
         rates = []
         for ri in range(min(steps, 3)):
@@ -133,15 +188,91 @@ def measure_plain(dev, cw, regs, nxt, steps):
             dt = time.perf_counter() - t0
             apps = s.kernel_stats()["mul_prime_apps"] - before["mul_prime_apps"]
             rates.append(apps / dt)
-            print(f"# plain region {regs[ri][0]} {dt * 1e3:.1f} ms {apps / dt:.4g} factor apps/s",
-                  file=sys.stderr, flush=True)
+            say(f"plain region {regs[ri][0]} {dt * 1e3:.1f} ms {apps / dt:.4g} factor apps/s")
         total = sum(F / rates[k] for k, (_, _, _, _, F) in enumerate(regs[:len(rates)]))
         if len(rates) < 3:
             mean = sum(rates) / len(rates)
             total += sum(F / mean for (_, _, _, _, F) in regs[len(rates):])
-        out = dict(value=327107 / total, region_rates_factor_apps_per_s=rates,
-                   structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
-    return out
+    return dict(value=327107 / total, region_rates_factor_apps_per_s=rates,
+                structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
+
+
+def measure_batch256(dev, cw, steps: int, tune: bool, target_apps: float = 4e12) -> dict:
+    """The 256-range batch: one batch (graph-replayed segment steps), windows
+    of the sieving-prime axis at the midpoints of `steps` equal quantiles of
+    the factor-application mass F(q) = sum over groups of tp_n x pi(min(q, T));
+    each bin's time = its mass / the rate measured in its window."""
+    ranges = batch_ranges()
+    t0 = time.perf_counter()
+    b = cw.Batch(dev, ranges, graph=True)
+    setup = time.perf_counter() - t0
+    groups = sorted((st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t])
+    ntp = sum(st.tpcount for st in b.info)
+    bands = {}
+    for st in b.info:
+        bands[st.limbs] = bands.get(st.limbs, 0) + 1
+    Ts = [T for T, _ in groups]
+    suffix = [0] * (len(groups) + 1)  # test primes of groups k.. (sorted by T)
+    for k in range(len(groups) - 1, -1, -1):
+        suffix[k] = suffix[k + 1] + groups[k][1]
+    below = [0.0] * (len(groups) + 1)
+    for k in range(len(groups)):
+        below[k + 1] = below[k] + groups[k][1] * pi_approx(Ts[k])
+
+    def F_below(q):
+        k = bisect.bisect_left(Ts, q)  # groups with T < q are complete
+        return below[k] + suffix[k] * pi_approx(q)
+
+    def active(q):
+        return suffix[bisect.bisect_left(Ts, q)]
+
+    F = F_below(Ts[-1])
+    say(f"batch256: {len(ranges)} ranges, {ntp} test primes, bands {bands}, F {F:.4g}, setup {setup:.1f}s")
+
+    def quantile(frac):
+        lo, hi = 2.0, float(Ts[-1])
+        for _ in range(100):
+            mid = math.sqrt(lo * hi)
+            if F_below(mid) < frac * F:
+                lo = mid
+            else:
+                hi = mid
+        return int(hi)
+
+    tune_rep = None
+    if tune:
+        t0 = time.perf_counter()
+        tune_rep = b.autotune(quantile(0.5))
+        say(f"batch256 autotune {time.perf_counter() - t0:.1f}s: "
+            + "; ".join(f"L={t['limbs']} per_lane {t['per_lane']} items/block {t['items_per_block']}" for t in tune_rep))
+    rates, windows = [], []
+    for k in range(steps):
+        q = quantile((k + 0.5) / steps)
+        w = int(target_apps * math.log(q) / max(1, active(q)))
+        w = max(61440, min(w, q // 4))
+        before = b.kernel_stats()
+        t0 = time.perf_counter()
+        b.run(q - w, q)
+        dt = time.perf_counter() - t0
+        ks = b.kernel_stats()
+        apps = ks["mul_prime_apps"] - before["mul_prime_apps"]
+        segs = ks["segments"] - before["segments"]
+        rates.append(apps / dt)
+        windows.append(dict(q=q, width=w, active_test_primes=active(q), seconds=dt, factor_apps=apps,
+                            segments=segs))
+        say(f"batch256 window {k}: [{q - w}, {q}) active {active(q)} {segs} segments {dt * 1e3:.0f} ms "
+            f"{apps / dt:.4g} factor apps/s")
+    t0 = time.perf_counter()
+    b.iterate()
+    b.finalize()
+    tail = time.perf_counter() - t0
+    b.close()
+    total = sum(F / steps / r for r in rates) + tail
+    return dict(value=ntp / total, unit="primes/s", ranges=len(ranges), test_primes=ntp, limb_bands=bands,
+                factor_applications_total=F, projected_full_search_s=total, setup_s=setup, autotune=tune_rep,
+                iterate_finalize_s=tail, windows=windows,
+                workload="256 disjoint ranges, starts log-uniform in [1e9, 1e14] (seed 1234), widths uniform "
+                         "in [1e5, 1e7] rounded to even; one batch, segment steps replayed from a HIP graph")
 
 
 def main() -> int:
@@ -151,10 +282,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", type=int, default=0)
+    ap.add_argument("--engine", choices=["batch", "session"], default="batch",
+                    help="batched engine (one-range batch) or the session path")
+    ap.add_argument("--mode", type=int, default=0, help="session product kernel (engine session)")
     ap.add_argument("--per-lane", type=int, default=0, help="test primes per lane (0 = library default)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="windows timed on the one-prime-per-launch path for the batched/plain ratio")
+    ap.add_argument("--batch-steps", type=int, default=4,
+                    help="windows timed on the 256-range batch (0 = skip it)")
+    ap.add_argument("--no-tune", action="store_true", help="batched engine without the autotuner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would search its own "
@@ -166,9 +302,11 @@ def main() -> int:
     dev = cw.Device(0)
     info = dev.info()
     t_setup = time.time()
-    sess = cw.Session(dev, PMIN, PMAX, mode=args.mode, per_lane=args.per_lane)
-    T = sess.type_targets
-    cnt = sess.tpcnt
+    if args.engine == "batch":
+        run = BatchRunner(cw, dev, args.per_lane, not args.no_tune)
+    else:
+        run = SessionRunner(cw, dev, args.mode, args.per_lane)
+    T, cnt, L = run.T, run.cnt, run.limbs
     regs = regions(T, cnt)
     # windows just below each region's upper end, disjoint across steps
     nxt = [r[2] for r in regs]
@@ -179,45 +317,39 @@ def main() -> int:
         nxt[ri] = lo
         return lo, hi
 
-    print(f"# setup {time.time() - t_setup:.1f}s: {sess.n} test primes {cnt}, targets {T}, limbs {sess.limbs}",
-          file=sys.stderr, flush=True)
+    say(f"setup {time.time() - t_setup:.1f}s: {sum(cnt)} test primes {cnt}, targets {T}, limbs {L}, {run.name}")
+    lib = cw.lib()
     for i in range(args.warmup):
         lo, hi = window(i % 3)
-        sess.run(lo, hi)
-    sess.reset_stats()
+        run.run(lo, hi)
+    lib.clw_finish(dev.ctx)
+    run.reset()
     step_ms = []
     region_time = [0.0, 0.0, 0.0]
     region_apps = [0.0, 0.0, 0.0]
-    mul_ms = mul_launch = rec_apps = 0.0
-    lib = cw.lib()
-    lib.clw_finish(dev.ctx)
     t_all = time.perf_counter()
     for i in range(args.steps):
         ri = i % 3
         lo, hi = window(ri)
-        before = sess.kernel_stats()
+        before = run.stats()
         t0 = time.perf_counter()
-        sess.run(lo, hi)
+        run.run(lo, hi)
         lib.clw_finish(dev.ctx)
         dt = time.perf_counter() - t0
-        after = sess.kernel_stats()
+        after = run.stats()
+        apps = after["mul_prime_apps"] - before["mul_prime_apps"]
         step_ms.append(dt * 1e3)
         region_time[ri] += dt
-        region_apps[ri] += after["mul_prime_apps"] - before["mul_prime_apps"]
-        print(f"# step {i} region {regs[ri][0]} [{lo}, {hi}) {dt * 1e3:.1f} ms "
-              f"{(after['mul_prime_apps'] - before['mul_prime_apps']) / dt:.4g} factor apps/s",
-              file=sys.stderr, flush=True)
+        region_apps[ri] += apps
+        say(f"step {i} region {regs[ri][0]} [{lo}, {hi}) {dt * 1e3:.1f} ms {apps / dt:.4g} factor apps/s")
     wall = time.perf_counter() - t_all
-    ks = sess.kernel_stats()
-    plain = None
-    if args.plain_steps > 0:
-        plain = measure_plain(dev, cw, regs, nxt, args.plain_steps)
+    ks = run.stats()
     # iterate + final check once (their share of a full search)
     t0 = time.perf_counter()
-    sess.iterate()
-    res = sess.finalize(allow_check_failure=True)  # windows only: rem != 0 expected
+    run.tail()
     t_tail = time.perf_counter() - t0
-    sess.close()
+    run.close()
+    plain = measure_plain(dev, cw, regs, nxt, args.plain_steps) if args.plain_steps > 0 else None
 
     rates = [region_apps[r] / region_time[r] if region_time[r] > 0 else None for r in range(3)]
     measured = [r for r in rates if r]
@@ -226,9 +358,9 @@ def main() -> int:
     value = sum(cnt) / total
 
     peak = info["compute_units"] * 64 * info["clock_khz"] * 1e3  # v_mad_u64_u32 / s
-    L = res.limbs
     prod_per_rec = 2 * L * L + L
     achieved = ks["mul_record_apps"] * prod_per_rec / (ks["mul_ms"] * 1e-3) if ks["mul_ms"] else 0.0
+    batch256 = measure_batch256(dev, cw, args.batch_steps, not args.no_tune) if args.batch_steps > 0 else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -243,19 +375,20 @@ def main() -> int:
         "dtype": "u32",
         "data": "exact: every prime in [2e13, 2e13+1e7); windows of the sieving-prime axis, extrapolated",
         "config": {
-            "workload": "config 3, plain method: every prime p in [2e13, 2e13+1e7), one range per launch",
+            "workload": "every prime p in [2e13, 2e13+1e7) (production magnitude, p^2 ~ 2^89)",
+            "engine": run.name,
             "test_primes": sum(cnt), "types": cnt, "type_targets": T, "limbs": L,
             "step": f"one segment of {SPAN} sieving integers for all test primes",
             "factor_applications_total": sum(r[4] for r in regs),
             "region_rates_factor_apps_per_s": {regs[k][0]: rates[k] for k in range(3)},
             "iterate_finalize_s": t_tail,
             "projected_full_search_s": total,
-            "mode": "lane per test prime" if args.mode in (0, 1) else "wave per test prime",
-            "per_lane": args.per_lane or int(cw.lib().clw_default_per_lane(L)),
+            "per_lane": run.kp,
+            "autotune": getattr(run, "tune", None),
         },
         "roofline": {
             "bound": "valu-int32-mad",
-            "kernel": f"k_mul_records<{L}>",
+            "kernel": run.kernel,
             "achieved": achieved / 1e12,
             "peak": peak / 1e12,
             "unit": "Tmad/s",
@@ -265,9 +398,10 @@ def main() -> int:
             "launches": ks["mul_launches"],
             "mean_launch_ms": ks["mul_ms"] / max(1, ks["mul_launches"]),
         },
-        "kernel_ms": {k: ks[k] for k in ("sieve_ms", "mul_ms", "side_ms", "reduce_ms")},
+        "kernel_ms": {k: ks[k] for k in ("sieve_ms", "mul_ms", "reduce_ms")},
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
+        "batch256": batch256,
         "wall_s": wall,
     }
     if not args.no_cpu:

@@ -71,6 +71,21 @@ class KernelStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BandTune(C.Structure):
+    _fields_ = [("limbs", C.c_uint32), ("per_lane", C.c_uint32), ("items_per_block", C.c_uint32),
+                ("trials", C.c_uint32), ("from_cache", C.c_uint32), ("active_test_primes", C.c_uint64),
+                ("best_apps_per_s", C.c_double), ("trial_per_lane", C.c_uint32 * 16),
+                ("trial_items_per_block", C.c_uint32 * 16), ("trial_apps_per_s", C.c_double * 16)]
+
+    def as_dict(self) -> dict:
+        n = min(self.trials, 16)
+        return dict(limbs=self.limbs, per_lane=self.per_lane, items_per_block=self.items_per_block,
+                    trials=self.trials, from_cache=bool(self.from_cache),
+                    active_test_primes=self.active_test_primes, best_apps_per_s=self.best_apps_per_s,
+                    candidates=[(self.trial_per_lane[k], self.trial_items_per_block[k], self.trial_apps_per_s[k])
+                                for k in range(n)])
+
+
 class WorkStatus(C.Structure):
     _fields_ = [("pmin", C.c_uint64), ("pmax", C.c_uint64), ("currp", C.c_uint64), ("trickle", C.c_uint64),
                 ("state_sum", C.c_uint64), ("totalcount", C.c_uint64), ("tpcount", C.c_uint32),
@@ -164,6 +179,7 @@ def lib():
         "clw_batch_finalize": (i32, [vp]),
         "clw_batch_results": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
         "clw_batch_kernel_stats": (i32, [vp, P(KernelStats)]),
+        "clw_batch_autotune": (i32, [vp, u64, C.c_char_p, P(BandTune), u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -465,6 +481,14 @@ class Batch:
         ks = KernelStats()
         check(lib().clw_batch_kernel_stats(self.h, C.byref(ks)))
         return ks.as_dict()
+
+    def autotune(self, q: int, cache_path: str | None = None) -> list[dict]:
+        """Per limb band: test primes per lane and items per resident block,
+        timed on a trial segment at magnitude q (no residue changes)."""
+        rep = (BandTune * 4)()
+        check(lib().clw_batch_autotune(self.h, q, cache_path.encode() if cache_path else None, rep, 4),
+              "batch_autotune")
+        return [rep[k].as_dict() for k in range(4) if rep[k].limbs]
 
 
 def ckpt_write(path: str, st: WorkStatus, values: list[int]) -> None:

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "clw_internal.h"
@@ -29,6 +31,23 @@ const uint64_t kLanesSpan = 16ull * kDefaultSpan;   // ~4.3e9 integers per segme
 const uint32_t kPlainGroups = 1024;                 // work-groups per plain launch
 
 // Odd primes below `lim` (simple sieve; lim is at most a few million).
+std::vector<uint32_t> small_odd_primes(uint32_t lim);
+
+// Odd primes up to at least `lim`, computed once per process and grown on
+// demand (the base of every host segmented sieve).
+std::shared_ptr<const std::vector<uint32_t>> base_primes(uint32_t lim) {
+  static std::mutex mu;
+  static std::shared_ptr<const std::vector<uint32_t>> cache;
+  static uint32_t cache_lim = 0;
+  std::lock_guard<std::mutex> g(mu);
+  if (!cache || cache_lim < lim) {
+    const uint32_t want = std::max(lim, 1u << 22);
+    cache = std::make_shared<const std::vector<uint32_t>>(small_odd_primes(want));
+    cache_lim = want;
+  }
+  return cache;
+}
+
 std::vector<uint32_t> small_odd_primes(uint32_t lim) {
   std::vector<uint8_t> comp(lim + 1, 0);
   std::vector<uint32_t> out;
@@ -61,9 +80,10 @@ int64_t signed_w(uint64_t p, uint64_t quot) {
 
 }  // namespace
 
-// Segmented sieve over [lo, hi) with trial primes up to min(sqrt(hi), 2^22);
-// when sqrt(hi) exceeds that, survivors are confirmed by deterministic
-// Miller-Rabin. Replaces primesieve_generate_primes (cl_wilson.cpp:1529).
+// Segmented sieve over [lo, hi) with base primes up to min(sqrt(hi), 2^25)
+// (complete for hi below 1.1e15); above, survivors are confirmed by
+// deterministic Miller-Rabin. Replaces primesieve_generate_primes
+// (cl_wilson.cpp:1529).
 extern "C" uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap) {
   if (hi <= lo) return 0;
   uint64_t cnt = 0;
@@ -75,9 +95,10 @@ extern "C" uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uin
   uint64_t s = lo < 3 ? 3 : (lo | 1);
   if (s >= hi) return cnt;
   uint64_t r = (uint64_t)clw::isqrt64(hi) + 1;
-  const uint32_t lim = (uint32_t)std::min<uint64_t>(r, 1u << 22);
+  const uint32_t lim = (uint32_t)std::min<uint64_t>(r, 1u << 25);
   const bool need_mr = r > lim;
-  const std::vector<uint32_t> base = small_odd_primes(lim);
+  const auto base_ptr = base_primes(lim);
+  const std::vector<uint32_t>& base = *base_ptr;
   const uint64_t SEG = 1u << 20;  // odd numbers per window
   std::vector<uint8_t> mark(SEG);
   for (uint64_t a = s; a < hi; a += 2 * SEG) {

@@ -13,7 +13,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 #include "clw_internal.h"
@@ -31,10 +34,19 @@ struct GroupHost {  // mirror of BatchGroup
   uint32_t tp_off, tp_n;
   uint64_t part_off;
 };
+struct PlanHost {  // mirror of BatchBand
+  uint32_t items_per_chunk, K;
+  uint32_t pad[2];
+};
+const uint64_t kNoPart = ~0ull;
 
 const uint64_t kMaxP = 0xFFFFFFFFFFFFFFFFull / 4;
 const uint64_t kMaxWidth = 10000000;
-const uint64_t kBatchSpan = 4370ull * CLW_BLOCK_SPAN;
+const uint32_t kMaxSpanBlocks = 4370;  // largest default segment (268 M integers)
+const uint32_t kMinSpanBlocks = 16;
+const double kAppsPerSegment = 2e12;   // factor applications aimed at per segment
+const uint32_t kItemsPerBlock = 8;     // work items per resident block (balance)
+const uint32_t kMaxChunks = 64;
 
 inline uint32_t prime_type(uint64_t p) { return p % 3 == 1 ? 0 : (p % 12 == 5 ? 1 : 2); }
 inline uint64_t type_divisor(uint32_t t) { return t == 0 ? 6 : (t == 1 ? 4 : 2); }
@@ -52,7 +64,12 @@ struct Band {
   uint64_t* from_d = nullptr;
   uint64_t* part_lo = nullptr;
   uint64_t* part_hi = nullptr;
-  uint32_t K = 1, items_per_chunk = 0;
+  PlanHost* plan_d = nullptr;
+  PlanHost plan{};
+  uint64_t part_cap = 0;   // chunk-product slots
+  uint32_t resident = 0;   // persistent grid of the product kernel
+  uint32_t items_per_block = kItemsPerBlock;
+  uint64_t active_tp = 0;  // test primes of the groups active in this window
   std::vector<uint64_t> quot, rem;
 };
 
@@ -83,8 +100,8 @@ struct clw_batch {
   clw_options opt{};
   std::vector<RangeHost> ranges;
   Band bands[6];  // by limb count
-  uint64_t maxT = 0, span = 0;
-  uint32_t max_blocks = 0;
+  uint64_t maxT = 0;
+  uint32_t max_blocks = 0;  // record-buffer slots (largest segment + 2)
   uint32_t* rec = nullptr;
   uint32_t* nrec = nullptr;
   uint32_t* flags = nullptr;
@@ -95,14 +112,20 @@ struct clw_batch {
   bool iterated = false, finalized = false;
   clw_kernel_stats ks{};
   uint64_t applied = 0;  // sieving primes applied (all windows)
+  // Eager mode: 4 events per segment (start, after sieve, after products,
+  // after reductions), read once per window.
+  std::vector<hipEvent_t> events;
+  uint32_t ev_used = 0;
+  double wall_ms = 0;  // host wall time of clw_batch_run calls
 
   ~clw_batch() {
+    for (hipEvent_t e : events) hipEventDestroy(e);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
     for (Band& b : bands) {
       if (!b.L) continue;
       clw_tp_table_free(ctx, &b.tab);
-      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.part_lo, b.part_hi};
+      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.part_lo, b.part_hi, b.plan_d};
       for (void* x : ptrs) clw_free(ctx, x);
     }
     void* ptrs[] = {rec, nrec, flags, totals, seg_d};
@@ -113,7 +136,8 @@ struct clw_batch {
 extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64_t* pmax, uint32_t nranges,
                                 const clw_options* opt, clw_batch** out) {
   if (!ctx || !pmin || !pmax || !out || nranges == 0) return clw_fail(CLW_ERR_ARG, "clw_batch_create: bad argument");
-  if (clwi_batch_seg_size() != sizeof(SegHost) || clwi_batch_group_size() != sizeof(GroupHost))
+  if (clwi_batch_seg_size() != sizeof(SegHost) || clwi_batch_group_size() != sizeof(GroupHost) ||
+      clwi_batch_band_size() != sizeof(PlanHost))
     return clw_fail(CLW_ERR_STATE, "batch descriptor layout mismatch");
   *out = nullptr;
   clw_batch* B = new clw_batch();
@@ -129,8 +153,25 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       return clw_fail(CLW_ERR_ARG, "range %u [%llu, %llu) invalid", r, (unsigned long long)R.pmin,
                       (unsigned long long)R.pmax);
     }
-    R.tp.resize(clw_host_primes(R.pmin, R.pmax, nullptr, 0));
-    clw_host_primes(R.pmin, R.pmax, R.tp.data(), R.tp.size());
+  }
+  {  // test primes of every range, on up to 16 host threads
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+      for (uint32_t r; (r = next.fetch_add(1)) < nranges;) {
+        RangeHost& R = B->ranges[r];
+        std::vector<uint64_t> v(clw_host_primes(R.pmin, R.pmax, nullptr, 0));
+        clw_host_primes(R.pmin, R.pmax, v.data(), v.size());
+        R.tp.swap(v);
+      }
+    };
+    const uint32_t nt = std::min<uint32_t>(nranges, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (uint32_t k = 1; k < nt; ++k) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+  }
+  for (uint32_t r = 0; r < nranges; ++r) {
+    RangeHost& R = B->ranges[r];
     if (R.tp.empty()) {
       delete B;
       return clw_fail(CLW_ERR_ARG, "range %u has no primes", r);
@@ -172,52 +213,44 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     }
   }
   // Segment buffers and descriptor.
-  B->span = B->opt.segment_span ? (B->opt.segment_span + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN * CLW_BLOCK_SPAN
-                                : kBatchSpan;
-  B->max_blocks = (uint32_t)(B->span / CLW_BLOCK_SPAN + 2);  // +1 slot for the primes 2, 3, 5
+  const uint32_t span_blocks =
+      B->opt.segment_span ? (uint32_t)std::min<uint64_t>((B->opt.segment_span + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN,
+                                                         kMaxSpanBlocks)
+                          : kMaxSpanBlocks;
+  B->max_blocks = span_blocks + 2;  // + the slot for the primes 2, 3, 5
   int rc = clw_alloc(ctx, (size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
   if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)B->max_blocks * 4, (void**)&B->nrec);
   if (rc == CLW_OK) rc = clw_alloc(ctx, 16, (void**)&B->flags);
-  if (rc == CLW_OK) rc = clw_alloc(ctx, 16, (void**)&B->totals);
+  if (rc == CLW_OK) rc = clw_alloc(ctx, 32, (void**)&B->totals);
   if (rc == CLW_OK) rc = clw_alloc(ctx, sizeof(SegHost), (void**)&B->seg_d);
   if (rc == CLW_OK) rc = clw_memset(ctx, B->flags, 0, 16);
-  if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, 16);
-  const uint32_t waves = B->opt.target_waves ? B->opt.target_waves : (uint32_t)ctx->prop.multiProcessorCount * 4 * 8;
+  if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, 32);
   for (Band& bd : B->bands) {
     if (!bd.L || rc != CLW_OK) continue;
     bd.per_lane = B->opt.per_lane ? B->opt.per_lane : clw_default_per_lane((uint32_t)bd.L);
     const uint32_t n = (uint32_t)bd.p.size();
-    // items: (group, tile); chunks K so that items * K covers the wave target
-    bd.items_per_chunk = 0;
-    for (uint32_t gi = 0; gi < bd.groups.size(); ++gi) {
-      const uint32_t tiles = (bd.groups[gi].tp_n + 256 * bd.per_lane - 1) / (256 * bd.per_lane);
-      for (uint32_t tile = 0; tile < tiles; ++tile) bd.item_gt.push_back((gi << 16) | tile);
-      bd.items_per_chunk += tiles;
-    }
-    if (bd.groups.size() >= 65536 || bd.items_per_chunk == 0) {
-      rc = clw_fail(CLW_ERR_ARG, "too many groups in a batch band");
+    if (bd.groups.size() >= 65536) {
+      rc = clw_fail(CLW_ERR_ARG, "too many ranges in one limb band of a batch (%zu groups)", bd.groups.size());
       break;
     }
-    bd.K = std::max<uint32_t>(1, std::min<uint32_t>(64, (2 * waves / 4 + bd.items_per_chunk - 1) / bd.items_per_chunk));
-    uint64_t off = 0;
-    for (GroupHost& g : bd.groups) {
-      g.part_off = off;
-      off += (uint64_t)bd.K * g.tp_n;
-    }
-    rc = clw_tp_table_alloc(ctx, n, (uint32_t)bd.L, &bd.tab);
+    size_t tiles = 0;  // at one test prime per lane (the most items any plan has)
+    for (const GroupHost& g : bd.groups) tiles += (g.tp_n + 255) / 256;
+    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(64ull * n, 1ull << 24));
+    rc = clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident);
+    if (rc == CLW_OK && B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
+    if (rc == CLW_OK) rc = clw_tp_table_alloc(ctx, n, (uint32_t)bd.L, &bd.tab);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tab.p, bd.p.data(), (size_t)n * 8, 1);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tab.type, bd.type.data(), (size_t)n * 4, 1);
     if (rc == CLW_OK) rc = clw_setup(ctx, &bd.tab, 0);
     if (rc == CLW_OK) rc = clw_alloc(ctx, bd.groups.size() * sizeof(GroupHost), (void**)&bd.groups_d);
-    if (rc == CLW_OK) rc = clw_h2d(ctx, bd.groups_d, bd.groups.data(), bd.groups.size() * sizeof(GroupHost), 1);
     if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)n * 4, (void**)&bd.tp_group_d);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tp_group_d, bd.tp_group.data(), (size_t)n * 4, 1);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, bd.item_gt.size() * 4, (void**)&bd.item_gt_d);
-    if (rc == CLW_OK) rc = clw_h2d(ctx, bd.item_gt_d, bd.item_gt.data(), bd.item_gt.size() * 4, 1);
+    if (rc == CLW_OK) rc = clw_alloc(ctx, tiles * 4, (void**)&bd.item_gt_d);
+    if (rc == CLW_OK) rc = clw_alloc(ctx, sizeof(PlanHost), (void**)&bd.plan_d);
     if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)n * 8, (void**)&bd.from_d);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.from_d, bd.from.data(), (size_t)n * 8, 1);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)off * 8, (void**)&bd.part_lo);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)off * 8, (void**)&bd.part_hi);
+    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)bd.part_cap * 8, (void**)&bd.part_lo);
+    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)bd.part_cap * 8, (void**)&bd.part_hi);
   }
   if (rc == CLW_OK) rc = clw_finish(ctx);
   if (rc != CLW_OK) {
@@ -251,17 +284,85 @@ extern "C" int clw_batch_info(const clw_batch* B, uint32_t r, clw_range_stats* s
   return CLW_OK;
 }
 
-static int batch_step(clw_batch* B, uint32_t extra_block) {
-  clw_ctx* ctx = B->ctx;
-  CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
-  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+// Work plan of a window whose first sieving prime is q0: the groups with
+// target >= q0 get items (their tiles), the others none; K chunks per
+// segment so that items x K covers kItemsPerBlock per resident block, within
+// the chunk-product slots. Returns the test primes active.
+static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
+  uint64_t act = 0;
   for (int band = 2; band <= 4; ++band) {
     Band& bd = B->bands[band];
     if (!bd.L) continue;
-    CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.items_per_chunk, bd.K, B->rec,
-                      B->nrec, B->seg_d, (uint32_t)(band - 2), extra_block, 0, bd.part_lo, bd.part_hi));
-    CK(clwi_batch_reduce(ctx, &bd.tab, bd.tp_group_d, bd.groups_d, bd.K, bd.part_lo, bd.part_hi));
+    bd.item_gt.clear();
+    bd.active_tp = 0;
+    for (uint32_t gi = 0; gi < bd.groups.size(); ++gi) {
+      const GroupHost& g = bd.groups[gi];
+      if (g.T < q0) continue;
+      const uint32_t tiles = (g.tp_n + 256 * bd.per_lane - 1) / (256 * bd.per_lane);
+      for (uint32_t tile = 0; tile < tiles; ++tile) bd.item_gt.push_back((gi << 16) | tile);
+      bd.active_tp += g.tp_n;
+    }
+    const uint32_t ipc = (uint32_t)bd.item_gt.size();
+    uint32_t K = 1;
+    if (ipc) {
+      const uint64_t want = ((uint64_t)bd.items_per_block * bd.resident + ipc - 1) / ipc;
+      const uint64_t cap = std::max<uint64_t>(1, bd.part_cap / bd.active_tp);
+      K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, cap, (uint64_t)kMaxChunks}));
+    }
+    uint64_t off = 0;
+    for (GroupHost& g : bd.groups) {
+      if (g.T < q0) {
+        g.part_off = kNoPart;
+      } else {
+        g.part_off = off;
+        off += (uint64_t)K * g.tp_n;
+      }
+    }
+    bd.plan.items_per_chunk = ipc;
+    bd.plan.K = K;
+    CK(clw_h2d(B->ctx, bd.groups_d, bd.groups.data(), bd.groups.size() * sizeof(GroupHost), 0));
+    if (ipc) CK(clw_h2d(B->ctx, bd.item_gt_d, bd.item_gt.data(), (size_t)ipc * 4, 0));
+    CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 1));
+    act += bd.active_tp;
   }
+  *active_total = act;
+  return CLW_OK;
+}
+
+static int mark(clw_batch* B, bool timed) {
+  if (!timed) return CLW_OK;
+  if (B->ev_used == B->events.size()) {
+    hipEvent_t e;
+    const hipError_t err = hipEventCreate(&e);
+    if (err != hipSuccess) return clw_fail_hip(err, "hipEventCreate");
+    B->events.push_back(e);
+  }
+  const hipError_t err = hipEventRecord(B->events[B->ev_used++], clw_stream(B->ctx));
+  return err == hipSuccess ? CLW_OK : clw_fail_hip(err, "hipEventRecord");
+}
+
+// One segment: advance the descriptor, sieve, products of every band, then
+// fold every band's chunk products. `timed` brackets the phases with events
+// (eager launches only: a captured graph holds no event records).
+static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
+  clw_ctx* ctx = B->ctx;
+  CK(mark(B, timed));
+  CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
+  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+  CK(mark(B, timed));
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
+    if (!bd.L) continue;
+    CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
+                      B->seg_d, (uint32_t)(band - 2), extra_block, bd.resident, bd.part_lo, bd.part_hi,
+                      B->totals + 2));
+  }
+  CK(mark(B, timed));
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
+    if (bd.L) CK(clwi_batch_reduce(ctx, &bd.tab, bd.tp_group_d, bd.groups_d, bd.plan_d, bd.part_lo, bd.part_hi));
+  }
+  CK(mark(B, timed));
   return CLW_OK;
 }
 
@@ -276,10 +377,20 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   if (qlo < 2) qlo = 2;
   if (qhi <= qlo) return CLW_OK;
   const uint64_t lo = std::max<uint64_t>(qlo, 7);
+  uint64_t active = 0;
+  CK(plan_window(B, qlo, &active));
+  uint64_t span_blocks = B->max_blocks - 2;
+  if (!B->opt.segment_span && active) {
+    // about kAppsPerSegment factor applications per segment (the prime
+    // density ~1/ln q), so a step stays near a second
+    const double per_block = (double)active * CLW_BLOCK_SPAN / std::log((double)std::max<uint64_t>(qhi, 16));
+    span_blocks = std::min<uint64_t>(span_blocks, std::max<uint64_t>(kMinSpanBlocks, (uint64_t)(kAppsPerSegment / per_block) + 1));
+  }
+  const uint64_t span = span_blocks * CLW_BLOCK_SPAN;
   // Host replica of k_seg_advance: the segments of [lo, qhi).
   std::vector<std::pair<uint64_t, uint64_t>> segs;
   for (uint64_t a = lo; a < qhi;) {
-    uint64_t b = a + B->span - (a % 30);
+    uint64_t b = a + span - (a % 30);
     if (b > qhi) b = qhi;
     segs.push_back({a, b});
     a = b;
@@ -289,9 +400,11 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   SegHost h{};
   h.lo = lo;
   h.hi = lo;  // the first advance yields [lo, ...)
-  h.span = B->span;
+  h.span = span;
   h.end = std::max(qhi, lo);
   CK(clw_h2d(ctx, B->seg_d, &h, sizeof(h), 1));
+  B->ev_used = 0;
+  const bool timed = !B->opt.graph;
   for (size_t s = 0; s < segs.size(); ++s) {
     // The first segment of a window that holds 2, 3, 5 carries them as an
     // extra record block (slot nblk); its step runs eagerly.
@@ -302,13 +415,13 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
                                            CLW_BLOCK_SPAN)
                               : 0;
       CK(clwi_small_primes_block(ctx, qlo, qhi, B->rec, B->nrec, nb));
-      CK(batch_step(B, 1));
+      CK(batch_step(B, 1, timed));
     } else if (B->opt.graph) {
       if (!B->graph_exec) {
         hipStream_t st = clw_stream(ctx);
         hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return clw_fail_hip(e, "hipStreamBeginCapture");
-        const int rc = batch_step(B, 0);
+        const int rc = batch_step(B, 0, false);
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(st, &g);
         if (rc != CLW_OK) return rc;
@@ -320,7 +433,7 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
       const hipError_t e = hipGraphLaunch(B->graph_exec, clw_stream(ctx));
       if (e != hipSuccess) return clw_fail_hip(e, "hipGraphLaunch");
     } else {
-      CK(batch_step(B, 0));
+      CK(batch_step(B, 0, timed));
     }
     B->ks.segments++;
     B->ks.sieve_integers += segs[s].second - segs[s].first;
@@ -328,17 +441,29 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   // Flags and the number of sieving primes applied in this window.
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));  // folds the last segment's count
   uint32_t flags[4] = {0, 0, 0, 0};
-  uint64_t tot[2] = {0, 0};
+  uint64_t tot[4] = {0, 0, 0, 0};
   CK(clw_d2h(ctx, flags, B->flags, 16, 0));
-  CK(clw_d2h(ctx, tot, B->totals, 16, 1));
+  CK(clw_d2h(ctx, tot, B->totals, 32, 1));
   if (flags[0]) return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded: flags %u", flags[0]);
   uint64_t smallc = 0;
   for (uint64_t q : {2ull, 3ull, 5ull}) smallc += (q >= qlo && q < qhi);
   B->applied += tot[1] + smallc;
-  CK(clw_memset(ctx, B->totals, 0, 16));
-  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  B->ks.mul_ms += sec * 1e3;  // whole-window wall time of the batched steps
-  B->ks.mul_launches += segs.size();
+  B->ks.mul_record_apps += tot[2];
+  B->ks.mul_prime_apps += tot[3];
+  CK(clw_memset(ctx, B->totals, 0, 32));
+  for (uint32_t k = 0; k + 4 <= B->ev_used; k += 4) {
+    float a = 0, b = 0, c = 0;
+    hipEventElapsedTime(&a, B->events[k], B->events[k + 1]);
+    hipEventElapsedTime(&b, B->events[k + 1], B->events[k + 2]);
+    hipEventElapsedTime(&c, B->events[k + 2], B->events[k + 3]);
+    B->ks.sieve_ms += a;
+    B->ks.mul_ms += b;
+    B->ks.reduce_ms += c;
+    B->ks.sieve_launches++;
+    B->ks.reduce_launches++;
+  }
+  for (int band = 2; band <= 4; ++band) B->ks.mul_launches += B->bands[band].L ? segs.size() : 0;
+  B->wall_ms += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() * 1e3;
   return CLW_OK;
 }
 
@@ -427,5 +552,173 @@ extern "C" uint64_t clw_batch_applied(const clw_batch* B) { return B ? B->applie
 extern "C" int clw_batch_kernel_stats(const clw_batch* B, clw_kernel_stats* out) {
   if (!B || !out) return clw_fail(CLW_ERR_ARG, "clw_batch_kernel_stats: null argument");
   *out = B->ks;
+  out->side_ms = B->wall_ms;  // a batch has no side lists: wall time instead
+  return CLW_OK;
+}
+
+// ---- autotuner -------------------------------------------------------------
+//
+// Per limb band, picks the test primes per lane of the product kernel (1, 2
+// or 4 Montgomery products in lockstep, i.e. the tile of 256 x KP test
+// primes a work-group carries) and the work items per resident block (the
+// chunk count K, i.e. the fan-in of the reduction that folds chunk products).
+// Each candidate is timed on one trial segment at the caller's magnitude q:
+// sieve, then the band's product kernel only, writing chunk products to
+// scratch, so no accumulator changes. Results can be kept in a text cache
+// keyed by device, band and the log2 of the band's active test primes.
+
+namespace {
+
+struct TuneKey {
+  char arch[64];
+  int cus, L, size_class;
+};
+
+bool cache_lookup(const char* path, const TuneKey& k, uint32_t* kp, uint32_t* ipb) {
+  if (!path) return false;
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  char arch[64];
+  int cus, L, sc;
+  unsigned a, b;
+  bool hit = false;
+  while (fscanf(f, "%63s %d %d %d %u %u", arch, &cus, &L, &sc, &a, &b) == 6) {
+    if (!strcmp(arch, k.arch) && cus == k.cus && L == k.L && sc == k.size_class) {
+      *kp = a;
+      *ipb = b;
+      hit = true;  // the last matching line wins
+    }
+  }
+  fclose(f);
+  return hit;
+}
+
+void cache_store(const char* path, const TuneKey& k, uint32_t kp, uint32_t ipb) {
+  if (!path) return;
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %d %u %u\n", k.arch, k.cus, k.L, k.size_class, kp, ipb);
+  fclose(f);
+}
+
+}  // namespace
+
+// One trial: the band's product kernel on the segment [q, q + span) of the
+// records, timed with events. Returns factor applications per second.
+static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, double* rate) {
+  clw_ctx* ctx = B->ctx;
+  uint64_t active = 0;
+  CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
+  if (B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
+  CK(plan_window(B, q, &active));
+  SegHost h{};
+  h.lo = q;
+  h.hi = q;
+  h.span = span_blocks * CLW_BLOCK_SPAN;
+  h.end = q + h.span;
+  CK(clw_h2d(ctx, B->seg_d, &h, sizeof(h), 1));
+  CK(clw_memset(ctx, B->totals, 0, 32));
+  CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
+  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, clw_stream(ctx));
+  int rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
+                          (uint32_t)(bd.L - 2), 0, bd.resident, bd.part_lo, bd.part_hi, B->totals + 2);
+  hipEventRecord(e1, clw_stream(ctx));
+  uint64_t tot[4] = {0, 0, 0, 0};
+  if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, 32, 1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  CK(rc);
+  CK(clw_memset(ctx, B->totals, 0, 32));
+  *rate = ms > 0 ? (double)tot[3] / (ms * 1e-3) : 0.0;
+  return CLW_OK;
+}
+
+extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_path, clw_band_tune* report,
+                                  uint32_t report_cap) {
+  if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_autotune: null batch");
+  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune after iterate");
+  clw_ctx* ctx = B->ctx;
+  if (q < 7) q = 7;
+  if (q > B->maxT) q = B->maxT;
+  uint32_t nrep = 0;
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
+    if (!bd.L) continue;
+    uint64_t active = 0;
+    for (const GroupHost& g : bd.groups) active += g.T >= q ? g.tp_n : 0;
+    clw_band_tune t{};
+    t.limbs = (uint32_t)bd.L;
+    t.active_test_primes = active;
+    TuneKey key{};
+    snprintf(key.arch, sizeof(key.arch), "%s", ctx->prop.gcnArchName);
+    for (char* c = key.arch; *c; ++c)
+      if (*c == ' ') *c = '_';
+    key.cus = ctx->prop.multiProcessorCount;
+    key.L = bd.L;
+    key.size_class = active ? 63 - __builtin_clzll(active) : 0;
+    uint32_t kp = 0, ipb = 0;
+    if (active == 0) {
+      // nothing of this band reaches q: keep its plan
+    } else if (cache_lookup(cache_path, key, &kp, &ipb)) {
+      bd.per_lane = kp;
+      bd.items_per_block = ipb;
+      t.from_cache = 1;
+    } else {
+      // trial segment of ~2e11 factor applications (a tenth of a step)
+      const double per_block = (double)active * CLW_BLOCK_SPAN / std::log((double)q);
+      const uint64_t blocks = std::min<uint64_t>(B->max_blocks - 2, std::max<uint64_t>(4, (uint64_t)(2e11 / per_block) + 1));
+      // candidates: test primes per lane with a k_batch_mul instantiation
+      static const uint32_t kps_by_band[5][6] = {{0}, {0}, {1, 2, 3, 4}, {2, 3, 4, 5, 6}, {1, 2, 3, 4}};
+      std::vector<uint32_t> kps;
+      for (uint32_t a : kps_by_band[bd.L])
+        if (a) kps.push_back(a);
+      const uint32_t ipbs[3] = {2, 8, 32};
+      double best = -1, warm = 0;
+      bd.per_lane = clw_default_per_lane((uint32_t)bd.L);
+      bd.items_per_block = kItemsPerBlock;
+      CK(tune_trial(B, bd, q, blocks, &warm));  // code-object load, clocks up
+      for (uint32_t a : kps) {
+        if (B->opt.per_lane && a != B->opt.per_lane) continue;
+        for (uint32_t b : ipbs) {
+          bd.per_lane = a;
+          bd.items_per_block = b;
+          double rate = 0;
+          CK(tune_trial(B, bd, q, blocks, &rate));
+          if (t.trials < 16) {
+            t.trial_per_lane[t.trials] = a;
+            t.trial_items_per_block[t.trials] = b;
+            t.trial_apps_per_s[t.trials] = rate;
+          }
+          t.trials++;
+          if (rate > best) {
+            best = rate;
+            kp = a;
+            ipb = b;
+          }
+        }
+      }
+      bd.per_lane = kp;
+      bd.items_per_block = ipb;
+      t.best_apps_per_s = best;
+      cache_store(cache_path, key, kp, ipb);
+    }
+    CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
+    if (B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
+    t.per_lane = bd.per_lane;
+    t.items_per_block = bd.items_per_block;
+    if (report && nrep < report_cap) report[nrep] = t;
+    nrep++;
+  }
+  // the product kernels may have changed: capture the segment step again
+  if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  B->graph_exec = nullptr;
+  B->graph = nullptr;
   return CLW_OK;
 }

@@ -29,14 +29,16 @@ int clw_upload_sieve_tables();
 extern "C" {
 int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uint32_t max_blocks, uint32_t* flags,
                      uint64_t* total);
+int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_lane, uint32_t* out);
 int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
-                   const uint32_t* item_gt, uint32_t items_per_chunk, uint32_t K, const uint32_t* rec,
-                   const uint32_t* nrec, void* seg, uint32_t qslot, uint32_t extra_block, uint32_t grid,
-                   uint64_t* part_lo, uint64_t* part_hi);
-int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups, uint32_t K,
-                      const uint64_t* part_lo, const uint64_t* part_hi);
+                   const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec, void* seg,
+                   uint32_t qslot, uint32_t extra_block, uint32_t grid, uint64_t* part_lo, uint64_t* part_hi,
+                   uint64_t* apps);
+int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
+                      const void* plan, const uint64_t* part_lo, const uint64_t* part_hi);
 int clwi_seg_advance(clw_ctx* ctx, void* seg, uint32_t* flags, uint64_t* total_prev, uint64_t* total);
 int clwi_small_primes_block(clw_ctx* ctx, uint64_t qlo, uint64_t qhi, uint32_t* rec, uint32_t* nrec, uint32_t blk);
 size_t clwi_batch_seg_size();
 size_t clwi_batch_group_size();
+size_t clwi_batch_band_size();
 }

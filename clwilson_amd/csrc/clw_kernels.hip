@@ -987,36 +987,60 @@ extern "C" int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t
   return clw_check_launch("sieve_pairs");
 }
 
-extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
-                              const uint32_t* item_gt, uint32_t items_per_chunk, uint32_t K, const uint32_t* rec,
-                              const uint32_t* nrec, void* seg, uint32_t qslot, uint32_t extra_block, uint32_t grid,
-                              uint64_t* part_lo, uint64_t* part_hi) {
-  // grid 0: as many resident blocks as the device holds (persistent kernel),
-  // at most one per work item.
-  const uint32_t n_items = items_per_chunk * K;
-  DISPATCH_LK(t->limbs, per_lane, {
-    uint32_t g = grid;
-    if (g == 0) {
-      int per_cu = 0;
-      const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_mul<LL, KK>, 256, 0);
-      if (e != hipSuccess) return clw_fail_hip(e, "occupancy query");
-      g = (uint32_t)(per_cu > 0 ? per_cu : 1) * (uint32_t)ctx->prop.multiProcessorCount;
-    }
-    if (g > n_items) g = n_items;
-    if (g == 0) return CLW_OK;
-    hipLaunchKernelGGL((k_batch_mul<LL, KK>), dim3(g), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
-                       (const BatchGroup*)groups, item_gt, items_per_chunk, K, rec, nrec, (BatchSeg*)seg, qslot,
-                       extra_block, part_lo, part_hi);
+// The batched product kernel's instantiations: limbs 2-4 x test primes per
+// lane (the autotuner's candidates).
+#define DISPATCH_BK(L, KP, CALL)                                                                          \
+  switch ((L) * 8 + (KP)) {                                                                               \
+    case 2 * 8 + 1: { constexpr int LL = 2, KK = 1; CALL; } break;                                        \
+    case 2 * 8 + 2: { constexpr int LL = 2, KK = 2; CALL; } break;                                        \
+    case 2 * 8 + 3: { constexpr int LL = 2, KK = 3; CALL; } break;                                        \
+    case 2 * 8 + 4: { constexpr int LL = 2, KK = 4; CALL; } break;                                        \
+    case 3 * 8 + 1: { constexpr int LL = 3, KK = 1; CALL; } break;                                        \
+    case 3 * 8 + 2: { constexpr int LL = 3, KK = 2; CALL; } break;                                        \
+    case 3 * 8 + 3: { constexpr int LL = 3, KK = 3; CALL; } break;                                        \
+    case 3 * 8 + 4: { constexpr int LL = 3, KK = 4; CALL; } break;                                        \
+    case 3 * 8 + 5: { constexpr int LL = 3, KK = 5; CALL; } break;                                        \
+    case 3 * 8 + 6: { constexpr int LL = 3, KK = 6; CALL; } break;                                        \
+    case 4 * 8 + 1: { constexpr int LL = 4, KK = 1; CALL; } break;                                        \
+    case 4 * 8 + 2: { constexpr int LL = 4, KK = 2; CALL; } break;                                        \
+    case 4 * 8 + 3: { constexpr int LL = 4, KK = 3; CALL; } break;                                        \
+    case 4 * 8 + 4: { constexpr int LL = 4, KK = 4; CALL; } break;                                        \
+    default: return clw_fail(CLW_ERR_ARG, "batch: unsupported limbs %u / primes per lane %u", (unsigned)(L), \
+                             (unsigned)(KP));                                                             \
+  }
+
+// Blocks of k_batch_mul<limbs, per_lane> resident on the whole device.
+extern "C" int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_lane, uint32_t* out) {
+  DISPATCH_BK(limbs, per_lane, {
+    int per_cu = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_mul<LL, KK>, 256, 0);
+    if (e != hipSuccess) return clw_fail_hip(e, "occupancy query");
+    *out = (uint32_t)(per_cu > 0 ? per_cu : 1) * (uint32_t)ctx->prop.multiProcessorCount;
   });
+  return CLW_OK;
+}
+
+// grid: the persistent grid (resident blocks, at most one per work item of
+// the largest plan); the kernel reads its plan from `plan`.
+extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
+                              const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec,
+                              void* seg, uint32_t qslot, uint32_t extra_block, uint32_t grid, uint64_t* part_lo,
+                              uint64_t* part_hi, uint64_t* apps) {
+  if (grid == 0) return CLW_OK;
+  DISPATCH_BK(t->limbs, per_lane,
+              hipLaunchKernelGGL((k_batch_mul<LL, KK>), dim3(grid), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
+                                 (const BatchGroup*)groups, item_gt, (const BatchBand*)plan, rec, nrec,
+                                 (BatchSeg*)seg, qslot, extra_block, part_lo, part_hi, (unsigned long long*)apps));
   return clw_check_launch("batch_mul");
 }
 
 extern "C" int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
-                                 uint32_t K, const uint64_t* part_lo, const uint64_t* part_hi) {
+                                 const void* plan, const uint64_t* part_lo, const uint64_t* part_hi) {
   if (t->count == 0) return CLW_OK;
   DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_batch_reduce<LL>, dim3(ceil_div(t->count, 256)), dim3(256), 0,
                                           clw_stream(ctx), CTX_ARGS(t), t->acc_lo, t->acc_hi, tp_group,
-                                          (const BatchGroup*)groups, t->count, K, part_lo, part_hi));
+                                          (const BatchGroup*)groups, t->count, (const BatchBand*)plan, part_lo,
+                                          part_hi));
   return clw_check_launch("batch_reduce");
 }
 
@@ -1034,3 +1058,4 @@ extern "C" int clwi_small_primes_block(clw_ctx* ctx, uint64_t qlo, uint64_t qhi,
 
 extern "C" size_t clwi_batch_seg_size() { return sizeof(BatchSeg); }
 extern "C" size_t clwi_batch_group_size() { return sizeof(BatchGroup); }
+extern "C" size_t clwi_batch_band_size() { return sizeof(BatchBand); }

@@ -310,7 +310,31 @@ int clw_batch_finalize(clw_batch* b);
  * totalcount is not tracked per range and reads 0) and per-prime outputs. */
 int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, uint64_t* out_p,
                       uint64_t* out_quot, uint64_t* out_rem, uint64_t out_cap);
-/* segments, sieve_integers and mul_ms (wall time of clw_batch_run). */
+/* Autotuner of the product kernel per limb band: times, on one trial
+ * segment of sieving primes at magnitude q (sieve + product kernel into
+ * scratch; no residue changes), every candidate of test primes per lane
+ * (1-4 for 2 and 4 limbs, 2-6 for 3 limbs: the number of Montgomery products
+ * a lane runs in lockstep, which sets the instruction schedule and the
+ * work-group tile of 256 x per_lane test primes) x work items per resident
+ * block (2, 8, 32:
+ * the chunk count, i.e. the fan-in of the chunk-product reduction), and keeps
+ * the fastest. With cache_path, a tuned (device, band, size class) is read
+ * from / appended to that text file instead of re-timed. report receives one
+ * entry per band present (at most report_cap). No counterpart in the
+ * reference, whose launch shapes are fixed (cl_wilson.cpp:1795-1843). */
+typedef struct {
+  uint32_t limbs, per_lane, items_per_block, trials, from_cache;
+  uint64_t active_test_primes;
+  double best_apps_per_s;
+  uint32_t trial_per_lane[16], trial_items_per_block[16];
+  double trial_apps_per_s[16];
+} clw_band_tune;
+int clw_batch_autotune(clw_batch* b, uint64_t q, const char* cache_path, clw_band_tune* report,
+                       uint32_t report_cap);
+/* segments, sieve_integers, record/prime applications (device counters);
+ * sieve_ms, mul_ms, reduce_ms from HIP events when the batch launches its
+ * steps eagerly (opt.graph = 0; 0 under graph replay); side_ms = host wall
+ * time of all clw_batch_run calls. */
 int clw_batch_kernel_stats(const clw_batch* b, clw_kernel_stats* out);
 
 /* ---- checkpoint / resume (cl_wilson.cpp:110-319, 1734-1777) ------------- */

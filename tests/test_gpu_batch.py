@@ -133,3 +133,50 @@ def test_batch_rejects_bad_ranges(dev):
         cw.Batch(dev, [(10**18, 10**18 + 100)])  # 5-limb band: sessions only
     with pytest.raises(cw.ClwError):
         cw.Batch(dev, [(100, 100 + 10**7 + 1)])
+
+
+def test_autotune_keeps_results(dev, tmp_path):
+    """The autotuner's trial segments leave every residue untouched and its
+    choice (whatever it is) gives the untuned results bit for bit; a second
+    tuning of the same shape comes from the cache."""
+    ranges = MIX[1:7]
+    with cw.Batch(dev, ranges) as b:
+        ref = b.search()
+    cache = str(tmp_path / "tune.txt")
+    with cw.Batch(dev, ranges) as b:
+        b.run(2, 1000)
+        before = [b.residues(r) for r in range(len(ranges))]
+        rep = b.autotune(5_000_000, cache)
+        say("autotune", rep)
+        assert rep and all(t["trials"] >= 3 and not t["from_cache"] for t in rep)
+        assert all(t["per_lane"] in (1, 2, 3, 4, 5, 6) for t in rep)
+        assert [b.residues(r) for r in range(len(ranges))] == before
+        b.run(1000, b.maxtarget + 1)
+        b.iterate()
+        b.finalize()
+        for r in range(len(ranges)):
+            _same(b.results(r), ref[r])
+    with cw.Batch(dev, ranges) as b:
+        rep2 = b.autotune(5_000_000, cache)
+        assert all(t["from_cache"] for t in rep2)
+        assert [(t["per_lane"], t["items_per_block"]) for t in rep2] == \
+            [(t["per_lane"], t["items_per_block"]) for t in rep]
+
+
+@pytest.mark.parametrize("per_lane", [1, 2, 3, 4, 5, 6])
+def test_batch_every_per_lane_variant(dev, per_lane):
+    """Every instantiation the autotuner may pick (test primes per lane)
+    gives the same results: 2 and 3 limbs by full searches, 4 limbs by window
+    residues against the session path."""
+    ranges = MIX[5:8] if per_lane > 4 else MIX[3:8]  # 5-6 per lane: 3 limbs only
+    say("per_lane", per_lane)
+    with cw.Batch(dev, ranges, per_lane=per_lane) as b:
+        got = b.search()
+    for (lo, hi), r in zip(ranges, got):
+        _same(r, dev.wilson_range(lo, hi))
+    if per_lane <= 4:
+        lo4, hi4 = 10**14, 10**14 + 3000
+        with cw.Batch(dev, [(lo4, hi4)], per_lane=per_lane) as b, cw.Session(dev, lo4, hi4) as s:
+            b.run(40000000, 900000000)
+            s.run(40000000, 900000000)
+            assert b.residues(0) == s.residues()
