@@ -94,7 +94,8 @@ class WorkStatus(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
-                ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32)]
+                ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32),
+                ("segment_apps", C.c_uint64), ("host_primes", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -168,6 +169,7 @@ def lib():
         "clw_plain_mul": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, vp, vp]),
         "clw_plain_reduce": (i32, [vp, P(TpTable), u32, u32, vp, vp]),
         "clw_iterate_from": (i32, [vp, P(TpTable), vp]),
+        "clw_device_primes": (i32, [vp, u64, u64, vp, u64, P(u64)]),
         "clw_batch_create": (i32, [vp, vp, vp, u32, P(Options), P(vp)]),
         "clw_batch_destroy": (i32, [vp]),
         "clw_batch_size": (u32, [vp]),
@@ -180,6 +182,13 @@ def lib():
         "clw_batch_results": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
         "clw_batch_kernel_stats": (i32, [vp, P(KernelStats)]),
         "clw_batch_autotune": (i32, [vp, u64, C.c_char_p, P(BandTune), u32]),
+        "clw_batch_launch": (i32, [vp, u64, u64]),
+        "clw_batch_wait": (i32, [vp]),
+        "clw_batch_query": (i32, [vp]),
+        "clw_batch_bind": (i32, [vp, vp]),
+        "clw_batch_covered": (u64, [vp]),
+        "clw_batch_device_bytes": (u64, [vp]),
+        "clw_batch_finish_range": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -280,6 +289,14 @@ class Device:
         check(lib().clw_get_device_info(self.ctx, C.byref(d)), "device_info")
         return dict(name=d.name.decode(), arch=d.arch.decode(), compute_units=d.compute_units,
                     clock_khz=d.clock_khz, global_mem=d.global_mem)
+
+    def primes(self, lo: int, hi: int) -> np.ndarray:
+        """Primes of [lo, hi) sieved on the device (the test-prime list)."""
+        cap = (hi - lo) // 2 + 4
+        out = np.zeros(cap, dtype=np.uint64)
+        n = C.c_uint64(0)
+        check(lib().clw_device_primes(self.ctx, lo, hi, out.ctypes.data, cap, C.byref(n)), "device_primes")
+        return out[: n.value]
 
     def wilson_range(self, pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
                      verbose: bool = False, allow_check_failure: bool = False, mode: int = 0,
@@ -400,7 +417,8 @@ class Batch:
     Each range's results equal clw_wilson_range's on it, bit for bit."""
 
     def __init__(self, dev: "Device", ranges: list[tuple[int, int]], segment_span: int = 0,
-                 target_waves: int = 0, per_lane: int = 0, graph: bool = True):
+                 target_waves: int = 0, per_lane: int = 0, graph: bool = True, segment_apps: int = 0,
+                 host_primes: bool = False):
         if not ranges:
             raise ValueError("a batch needs at least one range")
         self.dev = dev
@@ -408,7 +426,8 @@ class Batch:
         self.ranges = [(int(a), int(b)) for a, b in ranges]
         lo = np.array([a for a, _ in self.ranges], dtype=np.uint64)
         hi = np.array([b for _, b in self.ranges], dtype=np.uint64)
-        opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0)
+        opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0, segment_apps,
+                      1 if host_primes else 0)
         check(lib().clw_batch_create(dev.ctx, lo.ctypes.data, hi.ctypes.data, len(self.ranges), C.byref(opt),
                                      C.byref(self.h)), "batch_create")
         self.info = []
@@ -435,6 +454,37 @@ class Batch:
     def run(self, qlo: int, qhi: int) -> None:
         check(lib().clw_batch_run(self.h, qlo, qhi), f"batch_run [{qlo}, {qhi})")
 
+    def launch(self, qlo: int, qhi: int) -> None:
+        """Enqueue a window on this batch's stream (returns at once)."""
+        check(lib().clw_batch_launch(self.h, qlo, qhi), f"batch_launch [{qlo}, {qhi})")
+
+    def wait(self) -> None:
+        check(lib().clw_batch_wait(self.h), "batch_wait")
+
+    def bind(self, dev: "Device") -> None:
+        """Run this batch on another device context (stream) from now on."""
+        check(lib().clw_batch_bind(self.h, dev.ctx), "batch_bind")
+        self.dev = dev
+
+    def done(self) -> bool:
+        """True when the launched window has finished (never blocks)."""
+        rc = lib().clw_batch_query(self.h)
+        if rc < 0:
+            check(rc, "batch_query")
+        return rc == 1
+
+    def covered(self) -> int:
+        """Sieving primes [2, covered) have been applied to every range."""
+        return lib().clw_batch_covered(self.h)
+
+    def device_bytes(self) -> int:
+        return lib().clw_batch_device_bytes(self.h)
+
+    def finish_range(self, r: int, allow_check_failure: bool = False) -> RangeResult:
+        """Iterate and check range r alone once its sieving primes are all
+        applied (covered() > its largest target); the rest keep running."""
+        return self._results(r, lib().clw_batch_finish_range, allow_check_failure)
+
     def applied(self) -> int:
         return lib().clw_batch_applied(self.h)
 
@@ -452,12 +502,15 @@ class Batch:
         check(lib().clw_batch_finalize(self.h), "batch_finalize")
 
     def results(self, r: int, allow_check_failure: bool = False) -> RangeResult:
+        return self._results(r, lib().clw_batch_results, allow_check_failure)
+
+    def _results(self, r, fn, allow_check_failure) -> RangeResult:
         n = self.info[r].tpcount
         p = np.zeros(max(n, 1), dtype=np.uint64)
         q = np.zeros(max(n, 1), dtype=np.uint64)
         m = np.zeros(max(n, 1), dtype=np.uint64)
         st = RangeStats()
-        rc = lib().clw_batch_results(self.h, r, C.byref(st), p.ctypes.data, q.ctypes.data, m.ctypes.data, n)
+        rc = fn(self.h, r, C.byref(st), p.ctypes.data, q.ctypes.data, m.ctypes.data, n)
         if rc != CLW_OK and not (allow_check_failure and rc == CLW_ERR_CHECK):
             check(rc, f"batch_results {r}")
         pmin, pmax = self.ranges[r]

@@ -239,8 +239,17 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
   S->opt = opt ? *opt : clw_options{0, 0, 0, 0, 0};
   S->pmin = pmin;
   S->pmax = pmax;
-  S->tp.resize(clw_host_primes(pmin, pmax, nullptr, 0));
-  clw_host_primes(pmin, pmax, S->tp.data(), S->tp.size());
+  {
+    std::vector<uint64_t> v((pmax - pmin) / 2 + 4);
+    uint64_t n = 0;
+    const int rc = clw_device_primes(ctx, pmin, pmax, v.data(), v.size(), &n);
+    if (rc != CLW_OK) {
+      delete S;
+      return rc;
+    }
+    v.resize(n);
+    S->tp.swap(v);
+  }
   if (S->tp.empty()) {
     delete S;
     return clw_fail(CLW_ERR_ARG, "there are no primes to test in this range");

@@ -13,8 +13,8 @@
 #include <string.h>
 
 #include <algorithm>
-#include <cmath>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <thread>
 #include <vector>
@@ -47,6 +47,7 @@ const uint32_t kMinSpanBlocks = 16;
 const double kAppsPerSegment = 2e12;   // factor applications aimed at per segment
 const uint32_t kItemsPerBlock = 8;     // work items per resident block (balance)
 const uint32_t kMaxChunks = 64;
+const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 
 inline uint32_t prime_type(uint64_t p) { return p % 3 == 1 ? 0 : (p % 12 == 5 ? 1 : 2); }
 inline uint64_t type_divisor(uint32_t t) { return t == 0 ? 6 : (t == 1 ? 4 : 2); }
@@ -80,7 +81,29 @@ struct RangeHost {
   int band;
   uint32_t cnt[3] = {0, 0, 0};
   uint64_t T[3] = {0, 0, 0};
+  uint32_t band_off = 0;      // its test primes are [band_off, band_off + tp.size()) of the band table
+  bool finished = false;      // iterated and checked on its own (clw_batch_finish_range)
+  std::vector<uint64_t> quot, rem;  // by band-table order, once finished
 };
+
+// The table restricted to entries [off, off + n) (same arrays, offset).
+clw_tp_table table_view(const clw_tp_table& t, uint32_t off, uint32_t n) {
+  clw_tp_table v = t;
+  v.p += off;
+  v.type += off;
+  v.tp_target += off;
+  v.n_lo += off;
+  v.n_hi += off;
+  v.one_lo += off;
+  v.one_hi += off;
+  v.r2_lo += off;
+  v.r2_hi += off;
+  v.ninv += off;
+  v.acc_lo += off;
+  v.acc_hi += off;
+  v.count = n;
+  return v;
+}
 
 #define CK(x)                      \
   do {                             \
@@ -115,11 +138,21 @@ struct clw_batch {
   // Eager mode: 4 events per segment (start, after sieve, after products,
   // after reductions), read once per window.
   std::vector<hipEvent_t> events;
+  hipEvent_t done_ev = nullptr;  // recorded at the end of each launched window
   uint32_t ev_used = 0;
   double wall_ms = 0;  // host wall time of clw_batch_run calls
+  uint64_t next_q = 2;  // sieving primes [2, next_q) applied to every range
+  uint64_t bytes = 0;   // device memory held
+  struct {
+    bool active = false;
+    uint64_t qlo = 0, qhi = 0;
+    size_t nsegs = 0;
+    std::chrono::steady_clock::time_point t0;
+  } pending;            // window launched by clw_batch_launch, not yet waited for
 
   ~clw_batch() {
     for (hipEvent_t e : events) hipEventDestroy(e);
+    if (done_ev) hipEventDestroy(done_ev);
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     if (graph) hipGraphDestroy(graph);
     for (Band& b : bands) {
@@ -154,7 +187,7 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
                       (unsigned long long)R.pmax);
     }
   }
-  {  // test primes of every range, on up to 16 host threads
+  if (B->opt.host_primes) {  // test primes on up to 16 host threads
     std::atomic<uint32_t> next{0};
     auto work = [&]() {
       for (uint32_t r; (r = next.fetch_add(1)) < nranges;) {
@@ -169,6 +202,19 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     for (uint32_t k = 1; k < nt; ++k) pool.emplace_back(work);
     work();
     for (std::thread& t : pool) t.join();
+  } else {
+    for (uint32_t r = 0; r < nranges; ++r) {  // test primes sieved on the device
+      RangeHost& R = B->ranges[r];
+      std::vector<uint64_t> v((R.pmax - R.pmin) / 2 + 4);
+      uint64_t n = 0;
+      const int rc = clw_device_primes(ctx, R.pmin, R.pmax, v.data(), v.size(), &n);
+      if (rc != CLW_OK) {
+        delete B;
+        return rc;
+      }
+      v.resize(n);
+      R.tp.swap(v);
+    }
   }
   for (uint32_t r = 0; r < nranges; ++r) {
     RangeHost& R = B->ranges[r];
@@ -193,6 +239,7 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     Band& bd = B->bands[R.band];
     bd.L = R.band;
     R.pos.resize(R.tp.size());
+    R.band_off = (uint32_t)bd.p.size();
     for (uint32_t t = 0; t < 3; ++t) {
       if (!R.cnt[t]) continue;
       GroupHost g{};
@@ -218,11 +265,15 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
                                                          kMaxSpanBlocks)
                           : kMaxSpanBlocks;
   B->max_blocks = span_blocks + 2;  // + the slot for the primes 2, 3, 5
-  int rc = clw_alloc(ctx, (size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
-  if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)B->max_blocks * 4, (void**)&B->nrec);
-  if (rc == CLW_OK) rc = clw_alloc(ctx, 16, (void**)&B->flags);
-  if (rc == CLW_OK) rc = clw_alloc(ctx, 32, (void**)&B->totals);
-  if (rc == CLW_OK) rc = clw_alloc(ctx, sizeof(SegHost), (void**)&B->seg_d);
+  auto balloc = [&](size_t sz, void** ptr) {
+    B->bytes += sz;
+    return clw_alloc(ctx, sz, ptr);
+  };
+  int rc = balloc((size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
+  if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * 4, (void**)&B->nrec);
+  if (rc == CLW_OK) rc = balloc(16, (void**)&B->flags);
+  if (rc == CLW_OK) rc = balloc(32, (void**)&B->totals);
+  if (rc == CLW_OK) rc = balloc(sizeof(SegHost), (void**)&B->seg_d);
   if (rc == CLW_OK) rc = clw_memset(ctx, B->flags, 0, 16);
   if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, 32);
   for (Band& bd : B->bands) {
@@ -235,22 +286,23 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     }
     size_t tiles = 0;  // at one test prime per lane (the most items any plan has)
     for (const GroupHost& g : bd.groups) tiles += (g.tp_n + 255) / 256;
-    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(64ull * n, 1ull << 24));
+    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(16ull * n, 1ull << 24));
     rc = clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident);
     if (rc == CLW_OK && B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
     if (rc == CLW_OK) rc = clw_tp_table_alloc(ctx, n, (uint32_t)bd.L, &bd.tab);
+    B->bytes += (uint64_t)n * kTableBytesPerPrime;
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tab.p, bd.p.data(), (size_t)n * 8, 1);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tab.type, bd.type.data(), (size_t)n * 4, 1);
     if (rc == CLW_OK) rc = clw_setup(ctx, &bd.tab, 0);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, bd.groups.size() * sizeof(GroupHost), (void**)&bd.groups_d);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)n * 4, (void**)&bd.tp_group_d);
+    if (rc == CLW_OK) rc = balloc(bd.groups.size() * sizeof(GroupHost), (void**)&bd.groups_d);
+    if (rc == CLW_OK) rc = balloc((size_t)n * 4, (void**)&bd.tp_group_d);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tp_group_d, bd.tp_group.data(), (size_t)n * 4, 1);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, tiles * 4, (void**)&bd.item_gt_d);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, sizeof(PlanHost), (void**)&bd.plan_d);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)n * 8, (void**)&bd.from_d);
+    if (rc == CLW_OK) rc = balloc(tiles * 4, (void**)&bd.item_gt_d);
+    if (rc == CLW_OK) rc = balloc(sizeof(PlanHost), (void**)&bd.plan_d);
+    if (rc == CLW_OK) rc = balloc((size_t)n * 8, (void**)&bd.from_d);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.from_d, bd.from.data(), (size_t)n * 8, 1);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)bd.part_cap * 8, (void**)&bd.part_lo);
-    if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)bd.part_cap * 8, (void**)&bd.part_hi);
+    if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_lo);
+    if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_hi);
   }
   if (rc == CLW_OK) rc = clw_finish(ctx);
   if (rc != CLW_OK) {
@@ -258,6 +310,17 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     return rc;
   }
   *out = B;
+  return CLW_OK;
+}
+
+// Moves the batch to another context (stream) of the same device: later
+// windows, waits and finishes run there. No window may be in flight.
+extern "C" int clw_batch_bind(clw_batch* B, clw_ctx* ctx) {
+  if (!B || !ctx) return clw_fail(CLW_ERR_ARG, "clw_batch_bind: null argument");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_bind with a window in flight");
+  if (ctx->device != B->ctx->device) return clw_fail(CLW_ERR_ARG, "clw_batch_bind: another device");
+  CK(clw_finish(B->ctx));
+  B->ctx = ctx;
   return CLW_OK;
 }
 
@@ -366,11 +429,13 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   return CLW_OK;
 }
 
-// Applies every sieving prime q in [qlo, qhi) (clipped to [2, max target])
-// to every range of the batch. Windows must be disjoint across calls.
-extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
-  if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_run: null batch");
-  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_run after iterate");
+// Enqueues every sieving prime q in [qlo, qhi) (clipped to [2, max target])
+// for every range of the batch on the batch's stream; clw_batch_wait ends
+// the window. Windows must be disjoint across calls.
+extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
+  if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_launch: null batch");
+  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_launch after iterate");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_launch: previous window not waited for");
   clw_ctx* ctx = B->ctx;
   const auto t0 = std::chrono::steady_clock::now();
   if (qhi > B->maxT + 1) qhi = B->maxT + 1;
@@ -380,11 +445,12 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   uint64_t active = 0;
   CK(plan_window(B, qlo, &active));
   uint64_t span_blocks = B->max_blocks - 2;
-  if (!B->opt.segment_span && active) {
-    // about kAppsPerSegment factor applications per segment (the prime
-    // density ~1/ln q), so a step stays near a second
+  if (active) {
+    // about opt.segment_apps (default kAppsPerSegment) factor applications
+    // per segment at the prime density ~1/ln q, within the record buffer
+    const double apps = B->opt.segment_apps ? (double)B->opt.segment_apps : kAppsPerSegment;
     const double per_block = (double)active * CLW_BLOCK_SPAN / std::log((double)std::max<uint64_t>(qhi, 16));
-    span_blocks = std::min<uint64_t>(span_blocks, std::max<uint64_t>(kMinSpanBlocks, (uint64_t)(kAppsPerSegment / per_block) + 1));
+    span_blocks = std::min<uint64_t>(span_blocks, std::max<uint64_t>(kMinSpanBlocks, (uint64_t)(apps / per_block) + 1));
   }
   const uint64_t span = span_blocks * CLW_BLOCK_SPAN;
   // Host replica of k_seg_advance: the segments of [lo, qhi).
@@ -438,8 +504,41 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
     B->ks.segments++;
     B->ks.sieve_integers += segs[s].second - segs[s].first;
   }
-  // Flags and the number of sieving primes applied in this window.
-  CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));  // folds the last segment's count
+  // folds the last segment's prime count into the window total
+  CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
+  B->pending.active = true;
+  if (!B->done_ev) {
+    const hipError_t e = hipEventCreateWithFlags(&B->done_ev, hipEventDisableTiming);
+    if (e != hipSuccess) return clw_fail_hip(e, "hipEventCreate");
+  }
+  const hipError_t e = hipEventRecord(B->done_ev, clw_stream(ctx));
+  if (e != hipSuccess) return clw_fail_hip(e, "hipEventRecord");
+  B->pending.qlo = qlo;
+  B->pending.qhi = qhi;
+  B->pending.nsegs = segs.size();
+  B->pending.t0 = t0;
+  return CLW_OK;
+}
+
+// 1 if no window is in flight or the launched one has finished on the
+// device (clw_batch_wait then returns at once), 0 while it runs.
+extern "C" int clw_batch_query(const clw_batch* B) {
+  if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_query: null batch");
+  if (!B->pending.active || !B->done_ev) return 1;
+  const hipError_t e = hipEventQuery(B->done_ev);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return clw_fail_hip(e, "hipEventQuery");
+}
+
+// Waits for the launched window; checks the device flags and folds counters
+// and event times into the statistics.
+extern "C" int clw_batch_wait(clw_batch* B) {
+  if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_wait: null batch");
+  if (!B->pending.active) return CLW_OK;
+  B->pending.active = false;
+  clw_ctx* ctx = B->ctx;
+  const uint64_t qlo = B->pending.qlo, qhi = B->pending.qhi;
   uint32_t flags[4] = {0, 0, 0, 0};
   uint64_t tot[4] = {0, 0, 0, 0};
   CK(clw_d2h(ctx, flags, B->flags, 16, 0));
@@ -462,16 +561,76 @@ extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
     B->ks.sieve_launches++;
     B->ks.reduce_launches++;
   }
-  for (int band = 2; band <= 4; ++band) B->ks.mul_launches += B->bands[band].L ? segs.size() : 0;
-  B->wall_ms += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() * 1e3;
+  for (int band = 2; band <= 4; ++band) B->ks.mul_launches += B->bands[band].L ? B->pending.nsegs : 0;
+  B->wall_ms += std::chrono::duration<double>(std::chrono::steady_clock::now() - B->pending.t0).count() * 1e3;
+  if (qlo <= B->next_q && qhi > B->next_q) B->next_q = qhi;
   return CLW_OK;
+}
+
+extern "C" int clw_batch_run(clw_batch* B, uint64_t qlo, uint64_t qhi) {
+  CK(clw_batch_launch(B, qlo, qhi));
+  return clw_batch_wait(B);
+}
+
+// Sieving primes [2, next_q) have reached every range (windows from 2 on,
+// back to back).
+extern "C" uint64_t clw_batch_covered(const clw_batch* B) { return B ? B->next_q : 0; }
+
+extern "C" uint64_t clw_batch_device_bytes(const clw_batch* B) { return B ? B->bytes : 0; }
+
+static int range_results(const clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p, uint64_t* out_quot,
+                         uint64_t* out_rem, uint64_t cap);
+
+// One range done early: once the sieving primes up to its largest target
+// have been applied, iterate and check its test primes alone (the rest of
+// the batch keeps running) and return its results.
+extern "C" int clw_batch_finish_range(clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p,
+                                      uint64_t* out_quot, uint64_t* out_rem, uint64_t cap) {
+  if (!B || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_finish_range: bad argument");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_range with a window in flight");
+  RangeHost& R = B->ranges[r];
+  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_range after clw_batch_iterate");
+  if (!R.finished) {
+    const uint64_t mt = std::max(R.T[0], std::max(R.T[1], R.T[2]));
+    if (B->next_q <= mt)
+      return clw_fail(CLW_ERR_STATE, "range %u needs sieving primes up to %llu (covered below %llu)", r,
+                      (unsigned long long)mt, (unsigned long long)B->next_q);
+    Band& bd = B->bands[R.band];
+    const uint32_t n = (uint32_t)R.tp.size();
+    const clw_tp_table v = table_view(bd.tab, R.band_off, n);
+    uint64_t *dq = nullptr, *dr = nullptr;
+    int rc = clw_alloc(B->ctx, (size_t)n * 8, (void**)&dq);
+    if (rc == CLW_OK) rc = clw_alloc(B->ctx, (size_t)n * 8, (void**)&dr);
+    if (rc == CLW_OK) rc = clw_iterate_from(B->ctx, &v, bd.from_d + R.band_off);
+    if (rc == CLW_OK) rc = clw_finalize(B->ctx, &v, dq, dr, nullptr);
+    R.quot.assign(n, 0);
+    R.rem.assign(n, 0);
+    if (rc == CLW_OK) rc = clw_d2h(B->ctx, R.quot.data(), dq, (size_t)n * 8, 0);
+    if (rc == CLW_OK) rc = clw_d2h(B->ctx, R.rem.data(), dr, (size_t)n * 8, 1);
+    clw_free(B->ctx, dq);
+    clw_free(B->ctx, dr);
+    CK(rc);
+    R.finished = true;
+  }
+  return range_results(B, r, st, out_p, out_quot, out_rem, cap);
 }
 
 extern "C" int clw_batch_iterate(clw_batch* B) {
   if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_iterate: null batch");
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_iterate called twice");
-  for (Band& bd : B->bands)
-    if (bd.L) CK(clw_iterate_from(B->ctx, &bd.tab, bd.from_d));
+  bool any_finished = false;
+  for (const RangeHost& R : B->ranges) any_finished |= R.finished;
+  if (!any_finished) {
+    for (Band& bd : B->bands)
+      if (bd.L) CK(clw_iterate_from(B->ctx, &bd.tab, bd.from_d));
+  } else {
+    for (const RangeHost& R : B->ranges) {
+      if (R.finished) continue;
+      Band& bd = B->bands[R.band];
+      const clw_tp_table v = table_view(bd.tab, R.band_off, (uint32_t)R.tp.size());
+      CK(clw_iterate_from(B->ctx, &v, bd.from_d + R.band_off));
+    }
+  }
   CK(clw_finish(B->ctx));
   B->iterated = true;
   return CLW_OK;
@@ -499,19 +658,19 @@ extern "C" int clw_batch_finalize(clw_batch* B) {
   return CLW_OK;
 }
 
-// Per-range results after clw_batch_finalize, in increasing p; stats as
-// clw_session_finalize fills them (checksum, results, last result).
-extern "C" int clw_batch_results(const clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p,
-                                 uint64_t* out_quot, uint64_t* out_rem, uint64_t cap) {
-  if (!B || !st || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_results: bad argument");
-  if (!B->finalized) return clw_fail(CLW_ERR_STATE, "clw_batch_results before clw_batch_finalize");
+static int range_results(const clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p, uint64_t* out_quot,
+                         uint64_t* out_rem, uint64_t cap) {
+  if (!st) return clw_fail(CLW_ERR_ARG, "range results: null stats");
   CK(clw_batch_info(B, r, st));
   const RangeHost& R = B->ranges[r];
   const Band& bd = B->bands[R.band];
   int bad = 0;
   uint64_t checksum = 0;
   for (size_t k = 0; k < R.tp.size(); ++k) {
-    const uint64_t p = R.tp[k], quot = bd.quot[R.pos[k]], rem = bd.rem[R.pos[k]];
+    const uint32_t i = R.pos[k];
+    const uint64_t p = R.tp[k];
+    const uint64_t quot = R.finished ? R.quot[i - R.band_off] : bd.quot[i];
+    const uint64_t rem = R.finished ? R.rem[i - R.band_off] : bd.rem[i];
     if (rem != 0) bad = 1;
     if (clw_format_result(p, quot, nullptr, 0)) {
       st->resultcount++;
@@ -526,10 +685,20 @@ extern "C" int clw_batch_results(const clw_batch* B, uint32_t r, clw_range_stats
     }
   }
   st->checksum = checksum;
-  // sieving primes applied to this range: those up to its largest target
-  st->totalcount = 0;
+  st->totalcount = 0;  // not tracked per range in a batch
   if (bad) return clw_fail(CLW_ERR_CHECK, "Wilson quotient check failed in batch range %u", r);
   return CLW_OK;
+}
+
+// Per-range results after clw_batch_finalize (or clw_batch_finish_range), in
+// increasing p; stats as clw_session_finalize fills them (checksum, results,
+// last result).
+extern "C" int clw_batch_results(const clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p,
+                                 uint64_t* out_quot, uint64_t* out_rem, uint64_t cap) {
+  if (!B || !st || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_results: bad argument");
+  if (!B->finalized && !B->ranges[r].finished)
+    return clw_fail(CLW_ERR_STATE, "clw_batch_results before clw_batch_finalize");
+  return range_results(B, r, st, out_p, out_quot, out_rem, cap);
 }
 
 extern "C" int clw_batch_residues(clw_batch* B, uint32_t r, uint64_t* lo, uint64_t* hi, uint64_t cap) {

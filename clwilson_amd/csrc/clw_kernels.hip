@@ -14,6 +14,9 @@
 //   k_finalize     (p-1)! + 1 mod p^2 from the type identities -> quot, rem
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "clw_arith.h"
 #include "clw_internal.h"
 #include "clw_nt64.h"
@@ -218,6 +221,25 @@ __device__ uint32_t sieve_block(SieveLds& S, uint64_t blo, uint64_t lo, uint64_t
   }
 
   return np;
+}
+
+// The primes of [lo, hi) (lo >= 7) as a plain list: pass 1 (out == nullptr)
+// writes each block's count, pass 2 writes block b's primes, increasing, at
+// out[offset[b]...]. One block = CLW_BLOCK_SPAN integers from base + b*SPAN.
+__global__ __launch_bounds__(256) void k_sieve_list(uint64_t base, uint64_t lo, uint64_t hi, uint32_t nbases,
+                                                    uint32_t* __restrict__ counts,
+                                                    const uint64_t* __restrict__ offset, uint64_t* __restrict__ out,
+                                                    uint32_t* __restrict__ flags) {
+  __shared__ SieveLds S;
+  const uint32_t b = blockIdx.x;
+  const uint64_t blo = base + (uint64_t)b * CLW_BLOCK_SPAN;
+  const uint32_t np = sieve_block(S, blo, lo, hi, nbases, flags);
+  if (!out) {
+    if (threadIdx.x == 0) counts[b] = np;
+    return;
+  }
+  uint64_t* const dst = out + offset[b];
+  for (uint32_t i = threadIdx.x; i < np; i += 256) dst[i] = blo + S.prm[i];
 }
 
 // One block = CLW_BLOCK_SPAN consecutive integers from base + b*SPAN.
@@ -857,6 +879,66 @@ extern "C" int clw_sieve_segment(clw_ctx* ctx, const clw_seg_buffers* s, uint64_
                      s->side[1], s->side[2], s->side_count, s->flags, (unsigned long long*)s->total);
   if (nblocks) *nblocks = (uint32_t)nb;
   return clw_check_launch("sieve_pack");
+}
+
+// Test primes of [lo, hi) generated on the device in slices of at most
+// kListBlocks sieve blocks: count pass, host prefix sum, write pass.
+extern "C" int clw_device_primes(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap,
+                                 uint64_t* count) {
+  if (!ctx || !count || (cap && !out)) return clw_fail(CLW_ERR_ARG, "clw_device_primes: bad argument");
+  *count = 0;
+  if (hi <= lo) return CLW_OK;
+  uint64_t n = 0;
+  for (uint64_t p : {2ull, 3ull, 5ull})
+    if (p >= lo && p < hi) {
+      if (n < cap) out[n] = p;
+      ++n;
+    }
+  if (lo < 7) lo = 7;
+  const uint64_t kListBlocks = std::min<uint64_t>(1024, (hi - lo / 30 * 30) / CLW_BLOCK_SPAN + 1);
+  uint32_t *counts = nullptr, *flags = nullptr;
+  uint64_t *offs = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&counts, kListBlocks * 4);
+  if (e == hipSuccess) e = hipMalloc(&flags, 16);
+  if (e == hipSuccess) e = hipMalloc(&offs, kListBlocks * 8);
+  if (e == hipSuccess) e = hipMalloc(&dout, kListBlocks * PRIME_CAP * 8);
+  if (e == hipSuccess) e = hipMemsetAsync(flags, 0, 16, clw_stream(ctx));
+  int rc = e == hipSuccess ? CLW_OK : clw_fail_hip(e, "clw_device_primes: allocation");
+  std::vector<uint32_t> hc(kListBlocks);
+  std::vector<uint64_t> ho(kListBlocks);
+  for (uint64_t a = lo; rc == CLW_OK && a < hi;) {
+    const uint64_t base = a / 30 * 30;
+    const uint64_t b = std::min<uint64_t>(hi, base + kListBlocks * CLW_BLOCK_SPAN);
+    const uint32_t nb = (uint32_t)((b - base + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN);
+    const uint32_t nbases = (uint32_t)mr_bases_needed(b);
+    hipLaunchKernelGGL(k_sieve_list, dim3(nb), dim3(256), 0, clw_stream(ctx), base, a, b, nbases, counts,
+                       (const uint64_t*)nullptr, (uint64_t*)nullptr, flags);
+    rc = clw_check_launch("sieve_list");
+    if (rc == CLW_OK) rc = clw_d2h(ctx, hc.data(), counts, (size_t)nb * 4, 1);
+    uint64_t tot = 0;
+    for (uint32_t k = 0; k < nb; ++k) {
+      ho[k] = tot;
+      tot += hc[k];
+    }
+    if (rc == CLW_OK) rc = clw_h2d(ctx, offs, ho.data(), (size_t)nb * 8, 0);
+    if (rc == CLW_OK) {
+      hipLaunchKernelGGL(k_sieve_list, dim3(nb), dim3(256), 0, clw_stream(ctx), base, a, b, nbases, counts,
+                         (const uint64_t*)offs, dout, flags);
+      rc = clw_check_launch("sieve_list");
+    }
+    if (rc == CLW_OK && n < cap) rc = clw_d2h(ctx, out + n, dout, (size_t)std::min<uint64_t>(tot, cap - n) * 8, 1);
+    n += tot;
+    a = b;
+  }
+  uint32_t hflags = 0;
+  if (rc == CLW_OK) rc = clw_d2h(ctx, &hflags, flags, 4, 1);
+  if (rc == CLW_OK && hflags) rc = clw_fail(CLW_ERR_OVERFLOW, "clw_device_primes: sieve capacity exceeded");
+  hipFree(counts);
+  hipFree(flags);
+  hipFree(offs);
+  hipFree(dout);
+  *count = n;
+  return rc;
 }
 
 #define DISPATCH_LK(L, KP, CALL)                                                                          \

@@ -1,0 +1,372 @@
+"""Request loop: range workunits served by continuous batching on one GPU.
+
+The reference processes one workunit per process (BOINC hands it a range;
+main.cpp:313-405 runs cl_wilson() on it and exits). Here ranges arrive as a
+stream and are served together:
+
+* Admission. Queued requests are admitted in arrival order while the HBM
+  budget holds. An admission round (on its own thread and stream) splits
+  the requests it takes by the magnitude of their largest factorial target
+  (classes a factor 16 wide) and
+  turns each class into a cohort: a clw_batch whose accumulators stay
+  resident in HBM until its last range is done. Ranges of one cohort need
+  about the same sieving primes, so they share every window to the end. A
+  request that does not fit waits for memory to free.
+* Continuous batching. Cohorts admitted at different times run side by side
+  on `streams` HIP streams (one device context each): the event loop keeps a
+  window of sieving primes in flight on every stream, so the kernels of
+  different cohorts overlap on the GPU, and starts the next window on a
+  stream as soon as its last one is done (no round barrier).
+* Scheduling. Shortest work to the next completion first: the cohorts whose
+  next range needs the fewest factor applications to finish go first, and a
+  window stops at that range's largest factorial target, so a range finishes
+  (clw_batch_finish_range: iterate + check of its primes alone) as soon as
+  its sieving primes are in, while larger ranges of the same cohort go on.
+* Latency is completion time minus arrival time; each request's results are
+  those of a single-range run, bit for bit.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Batch, Device, RangeResult, host_primes
+from .workloads import Request, pi_approx
+
+BLOCK_SPAN = 61440
+
+
+@dataclass
+class Completion:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    result: RangeResult
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Cohort:
+    batch: Batch
+    slot: int
+    reqs: list[Request]
+    maxt: list[int]                      # largest factorial target per range
+    T: np.ndarray                        # group targets (one group per range and type)
+    n: np.ndarray                        # test primes per group
+    admitted: float
+    done: set = field(default_factory=set)
+    bytes: int = 0
+    rate: float = 0.0                    # factor applications/s it has been running at
+    window: tuple = ()                   # (lo, hi, launch time, counter) of the window in flight
+
+    def __post_init__(self):
+        self.piT = np.array([pi_approx(float(t)) for t in self.T])
+        self.work = self.work_to_next()  # refreshed whenever the cohort moves
+
+    def covered(self) -> int:
+        return self.batch.covered()
+
+    def active(self, q: int) -> int:
+        """Test primes whose targets reach q."""
+        return int(self.n[self.T >= q].sum())
+
+    def next_target(self) -> int | None:
+        c = self.covered()
+        pending = [m for r, m in enumerate(self.maxt) if r not in self.done and m >= c]
+        return min(pending) if pending else None
+
+    def work_to_next(self) -> float:
+        """Factor applications until the next range can finish."""
+        m = self.next_target()
+        if m is None:
+            return 0.0
+        c = self.covered()
+        sel = self.T >= c
+        return float((self.n[sel] * (np.minimum(self.piT[sel], pi_approx(m)) - pi_approx(c))).sum())
+
+
+class Server:
+    """Continuous-batching request loop over the batched engine.
+
+    Two threads: the admission thread builds cohorts (host test-prime sieve
+    on worker threads, tables, setup) on its own stream every
+    `admit_interval` s while the event loop keeps a window in flight on each
+    of `streams` streams; built cohorts are bound to a stream and join the
+    schedule at once. streams + 1 stays within the 4 hardware queues HIP
+    gives a process by default, so no stream waits behind another's kernel
+    in a shared queue."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.05,
+                 apps_per_second: float = 2e12, segment_blocks: int = 64, max_cohort: int = 64,
+                 admit_interval: float = 0.5):
+        self.devices = [Device(device) for _ in range(streams)]
+        self.admit_dev = Device(device)
+        self.budget = hbm_budget
+        self.step_seconds = step_seconds
+        self.rate = apps_per_second
+        self.segment_span = segment_blocks * BLOCK_SPAN
+        self.max_cohort = max_cohort
+        self.admit_interval = admit_interval
+        self.lock = threading.Lock()
+        self.queue: list[Request] = []
+        self.ready: list[_Cohort] = []       # built, not yet bound to a stream
+        self.cohorts: list[_Cohort] = []
+        self.running: list[_Cohort | None] = [None] * streams
+        self.used = 0                        # device bytes of built cohorts
+        self.reserved = 0                    # estimated bytes of cohorts being built
+        self.peak_used = 0
+        self.completed: list[Completion] = []
+        self.factor_apps = 0
+        self.windows = 0
+        self.admit_seconds = 0.0
+        self._slot = 0
+        self._stop = threading.Event()
+        self._thread: threading.Thread | None = None
+        self._error: BaseException | None = None
+        self.t0 = time.perf_counter()
+
+    def close(self) -> None:
+        self.stop_admission()
+        for c in self.cohorts + self.ready:
+            c.batch.wait()
+            c.batch.close()
+        self.cohorts, self.ready = [], []
+        for d in self.devices + [self.admit_dev]:
+            d.close()
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def submit(self, req: Request) -> None:
+        with self.lock:
+            self.queue.append(req)
+
+    # -- admission (its own thread) ---------------------------------------------
+    @staticmethod
+    def estimate_bytes(req: Request) -> int:
+        """Upper estimate of a request's device footprint: ~1.1 width/ln p
+        test primes x (table 88 B + plan and chunk-product slots)."""
+        n = 1.1 * (req.pmax - req.pmin) / math.log(max(req.pmin, 3)) + 64
+        return int(n * 400)
+
+    @staticmethod
+    def magnitude_class(req: Request) -> int:
+        """Classes of the largest factorial target (~pmax/2), a factor 16 wide."""
+        return int(math.log2(max(req.pmax // 2, 2)) / 4)
+
+    def admit(self) -> int:
+        """One admission round; returns the number of requests admitted."""
+        fixed = (self.segment_span // BLOCK_SPAN + 2) * 4096 * 32 + (1 << 20)
+        classes: dict[int, list[Request]] = {}
+        est: dict[int, int] = {}
+        with self.lock:
+            while self.queue:
+                r = self.queue[0]
+                k = self.magnitude_class(r)
+                grp = classes.get(k)
+                if grp is not None and len(grp) >= self.max_cohort:
+                    break
+                e = self.estimate_bytes(r) + (0 if grp else fixed)
+                if self.used + self.reserved + e > self.budget:
+                    break
+                classes.setdefault(k, []).append(self.queue.pop(0))
+                est[k] = est.get(k, 0) + e
+                self.reserved += e
+        t = time.perf_counter()
+        n = 0
+        for k in sorted(classes):
+            take = classes[k]
+            b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
+                      segment_apps=int(self.rate * self.step_seconds), host_primes=True)
+            groups = [(st.typeTarget[t_], st.tpcnt[t_]) for st in b.info for t_ in range(3) if st.tpcnt[t_]]
+            c = _Cohort(b, -1, take, [st.maxtarget for st in b.info],
+                        np.array([g[0] for g in groups], dtype=np.float64),
+                        np.array([g[1] for g in groups], dtype=np.int64), self.now())
+            c.bytes = b.device_bytes()
+            c.rate = self.rate / 8  # factor applications/s, refined by every window it runs
+            n += len(take)
+            with self.lock:
+                self.ready.append(c)
+                self.used += c.bytes
+                self.reserved -= est[k]
+                self.peak_used = max(self.peak_used, self.used)
+        self.admit_seconds += time.perf_counter() - t
+        return n
+
+    def _admission_loop(self) -> None:
+        try:
+            while not self._stop.is_set():
+                if self.queue:
+                    self.admit()
+                self._stop.wait(self.admit_interval)
+        except BaseException as e:  # surfaced by the event loop
+            self._error = e
+
+    def start_admission(self) -> None:
+        if self._thread is None:
+            self._stop.clear()
+            self._thread = threading.Thread(target=self._admission_loop, daemon=True)
+            self._thread.start()
+
+    def stop_admission(self) -> None:
+        if self._thread is not None:
+            self._stop.set()
+            self._thread.join()
+            self._thread = None
+
+    # -- windows (event loop) -----------------------------------------------------
+    def _window(self, c: _Cohort) -> tuple[int, int]:
+        lo = c.covered()
+        nxt = c.next_target()
+        act = max(1, c.active(lo))
+        w = int(c.rate * self.step_seconds * math.log(max(lo, 16)) / act) + 1
+        if lo < (1 << 20):
+            # small sieving primes carry large exponents (a power per prime):
+            # their cost is not proportional to the count, so go by bounds
+            w = max(w, (1 << 16) - lo) if lo < (1 << 16) else min(max(w, lo), (1 << 20) - lo)
+        return lo, min(lo + max(w, 1), nxt + 1)
+
+    def _bind_ready(self) -> None:
+        with self.lock:
+            ready, self.ready = self.ready, []
+        for c in ready:
+            # the stream with the least queued work
+            load = [sum(1 for x in self.cohorts if x.slot == s) for s in range(len(self.devices))]
+            c.slot = min(range(len(self.devices)), key=lambda s: load[s])
+            c.batch.bind(self.devices[c.slot])
+            self.cohorts.append(c)
+
+    def _launch(self, slot: int) -> bool:
+        idle = [c for c in self.cohorts if c.slot == slot and c.next_target() is not None]
+        if not idle:
+            return False
+        c = min(idle, key=lambda c: c.work)
+        lo, hi = self._window(c)
+        c.window = (lo, hi, time.perf_counter(), c.batch.kernel_stats()["mul_prime_apps"])
+        c.batch.launch(lo, hi)
+        self.running[slot] = c
+        return True
+
+    def _complete(self, slot: int) -> int:
+        c = self.running[slot]
+        c.batch.wait()
+        self.running[slot] = None
+        self.windows += 1
+        lo, hi, t, before = c.window
+        apps = c.batch.kernel_stats()["mul_prime_apps"] - before
+        self.factor_apps += apps
+        dt = time.perf_counter() - t
+        if lo >= (1 << 20) and dt > 0 and apps > 0:
+            c.rate = 0.5 * c.rate + 0.5 * apps / dt
+        done = 0
+        cov = c.covered()
+        for r, m in enumerate(c.maxt):
+            if r not in c.done and m < cov:
+                res = c.batch.finish_range(r)
+                c.done.add(r)
+                self.completed.append(Completion(c.reqs[r].rid, c.reqs[r].arrival, c.admitted, self.now(), res))
+                done += 1
+        if len(c.done) == len(c.reqs):
+            c.batch.close()
+            with self.lock:
+                self.used -= c.bytes
+            self.cohorts.remove(c)
+        else:
+            c.work = c.work_to_next()
+        return done
+
+    def poll(self) -> bool:
+        """One pass of the event loop: finished windows completed, built
+        cohorts bound, a new window on every idle stream. Returns whether
+        anything happened."""
+        if self._error is not None:
+            raise self._error
+        busy = False
+        if self.ready:
+            self._bind_ready()
+            busy = True
+        for slot in range(len(self.running)):
+            c = self.running[slot]
+            if c is not None and c.batch.done():
+                self._complete(slot)
+                busy = True
+            if self.running[slot] is None:
+                busy |= self._launch(slot)
+        return busy
+
+    def drain(self) -> None:
+        """Waits for every window in flight (used before shutting down)."""
+        for slot, c in enumerate(self.running):
+            if c is not None:
+                self._complete(slot)
+
+    def idle(self) -> bool:
+        with self.lock:
+            return not (self.queue or self.ready or self.cohorts or self.reserved)
+
+    def serve(self, requests: list[Request], duration: float, progress=None, until_done: bool = False) -> dict:
+        """Replays `requests` in real time (arrival seconds after start) for
+        `duration` seconds of wall clock (or until every request is done,
+        with until_done) and reports latency and throughput."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        k = 0
+        self.t0 = time.perf_counter()
+        self.start_admission()
+        last = 0.0
+        try:
+            while True:
+                t = self.now()
+                if t >= duration:
+                    break
+                if until_done and k == len(pending) and self.idle():
+                    break
+                while k < len(pending) and pending[k].arrival <= t:
+                    self.submit(pending[k])
+                    k += 1
+                if not self.poll():
+                    time.sleep(0.0005)
+                if progress and self.now() - last >= 10:
+                    last = self.now()
+                    progress(self.report(k))
+        finally:
+            self.stop_admission()
+            self.drain()
+        return self.report(k)
+
+    def report(self, arrived: int) -> dict:
+        lat = sorted(c.latency for c in self.completed)
+
+        def pct(p):
+            if not lat:
+                return None
+            return lat[min(len(lat) - 1, int(math.ceil(p / 100 * len(lat))) - 1)]
+
+        wall = self.now()
+        tested = sum(len(c.result.p) for c in self.completed)
+        with self.lock:
+            queued = len(self.queue)
+        return dict(wall_s=wall, arrived=arrived, admitted=arrived - queued,
+                    completed=len(self.completed),
+                    in_flight=sum(len(c.reqs) - len(c.done) for c in self.cohorts + self.ready),
+                    queued=queued, p50_latency_s=pct(50), p99_latency_s=pct(99),
+                    primes_tested=tested, primes_tested_per_s=tested / wall if wall else 0.0,
+                    factor_apps_per_s=self.factor_apps / wall if wall else 0.0, windows=self.windows,
+                    admission_thread_s=self.admit_seconds, streams=len(self.devices),
+                    hbm_budget=self.budget, hbm_peak=self.peak_used, cohorts=len(self.cohorts))
+
+
+def alone(dev: Device, req: Request) -> RangeResult:
+    """The same request served by itself (a one-range batch)."""
+    with Batch(dev, [(req.pmin, req.pmax)]) as b:
+        return b.search()[0]
+
+
+__all__ = ["Server", "Completion", "alone", "host_primes"]

@@ -224,6 +224,13 @@ typedef struct {
   uint32_t per_lane;       /* test primes per lane in clw_mul_records (0 auto) */
   uint32_t graph;          /* batches: replay the segment step as a captured
                               HIP graph (0 = launch it eagerly)               */
+  uint64_t segment_apps;   /* batches: factor applications aimed at per
+                              segment step (0 = 2e12, about a second); the
+                              segment never exceeds segment_span (batches:
+                              the record buffer, 0 = 268,492,800 integers)   */
+  uint32_t host_primes;    /* batches: test primes from the host sieve on
+                              worker threads (no device work while building,
+                              e.g. beside busy streams) instead of the device */
 } clw_options;
 
 typedef struct {
@@ -294,19 +301,40 @@ typedef struct clw_batch clw_batch;
 int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64_t* pmax, uint32_t nranges,
                      const clw_options* opt, clw_batch** out);
 int clw_batch_destroy(clw_batch* b);
+/* Moves the batch to another context (stream) of the same device; no window
+ * may be in flight. Lets one thread build batches while others run. */
+int clw_batch_bind(clw_batch* b, clw_ctx* ctx);
 uint32_t clw_batch_size(const clw_batch* b);
 /* Counts and targets of range r (as clw_session_info). */
 int clw_batch_info(const clw_batch* b, uint32_t r, clw_range_stats* stats);
 /* Applies the sieving primes q in [qlo, qhi), clipped to [2, the largest
  * target of the batch], to every range (disjoint windows across calls). */
 int clw_batch_run(clw_batch* b, uint64_t qlo, uint64_t qhi);
+/* clw_batch_run in two halves: launch enqueues the window on the batch's
+ * stream and returns; wait blocks until it is done and checks the device
+ * flags. Batches on different contexts (streams) overlap on the device. */
+int clw_batch_launch(clw_batch* b, uint64_t qlo, uint64_t qhi);
+int clw_batch_wait(clw_batch* b);
+/* 1 when the launched window has finished on the device (or none is in
+ * flight), 0 while it runs; never blocks. */
+int clw_batch_query(const clw_batch* b);
+/* End of the prefix [2, covered) of sieving primes applied to every range. */
+uint64_t clw_batch_covered(const clw_batch* b);
+/* Device memory held by the batch (tables, buffers, scratch). */
+uint64_t clw_batch_device_bytes(const clw_batch* b);
+/* Range r done early: once clw_batch_covered() exceeds its largest target,
+ * iterates and checks its test primes alone, leaving the others running, and
+ * returns its results as clw_batch_results does. */
+int clw_batch_finish_range(clw_batch* b, uint32_t r, clw_range_stats* stats, uint64_t* out_p,
+                           uint64_t* out_quot, uint64_t* out_rem, uint64_t out_cap);
 /* Sieving primes applied so far (all windows). */
 uint64_t clw_batch_applied(const clw_batch* b);
 /* Canonical running residues of range r, in increasing p. */
 int clw_batch_residues(clw_batch* b, uint32_t r, uint64_t* lo, uint64_t* hi, uint64_t cap);
 int clw_batch_iterate(clw_batch* b);
 int clw_batch_finalize(clw_batch* b);
-/* Range r after clw_batch_finalize: stats (checksum, results, last result;
+/* Range r after clw_batch_finalize or clw_batch_finish_range: stats
+ * (checksum, results, last result;
  * totalcount is not tracked per range and reads 0) and per-prime outputs. */
 int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, uint64_t* out_p,
                       uint64_t* out_quot, uint64_t* out_rem, uint64_t out_cap);
@@ -374,6 +402,12 @@ int clw_range_types(uint64_t pmin, uint64_t pmax, uint32_t tpcnt[3], uint64_t ta
 /* Exact primes in [lo, hi) on the host (the primesieve_generate_primes call of
  * cl_wilson.cpp:1529). Returns the count; writes at most cap primes. */
 uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap);
+/* The same list generated on the device by the segment sieve (wheel-30 trial
+ * division in LDS, then deterministic Miller-Rabin): the test primes of
+ * sessions and batches. *count = number of primes (those beyond cap are
+ * counted, not written). Replaces primesieve_generate_primes for the test
+ * primes (cl_wilson.cpp:1529). */
+int clw_device_primes(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap, uint64_t* count);
 
 #ifdef __cplusplus
 }

@@ -180,3 +180,11 @@ def test_batch_every_per_lane_variant(dev, per_lane):
             b.run(40000000, 900000000)
             s.run(40000000, 900000000)
             assert b.residues(0) == s.residues()
+
+
+def test_batch_host_prime_source(dev):
+    """Test primes from the host sieve (worker threads) or the device sieve
+    build the same batch and give the same results."""
+    with cw.Batch(dev, MIX[2:8], host_primes=True) as a, cw.Batch(dev, MIX[2:8]) as b:
+        for x, y in zip(a.search(), b.search()):
+            _same(x, y)

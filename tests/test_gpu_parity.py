@@ -157,3 +157,11 @@ def test_selftest_large_primes(dev, golden, k):
     res = dev.wilson_range(kat["pmin"], kat["pmax"])
     say("kat", k + 1, round(res.seconds, 2))
     _kat_check(res, kat)
+
+
+@pytest.mark.parametrize("lo,hi", [(2, 100), (5, 20000), (7, 8), (61439, 61441), (1_000_000, 3_000_000),
+                                   (2**32 - 5000, 2**32 + 5000), (10**14, 10**14 + 200_000),
+                                   (2**62 - 20_000, 2**62), (759_250_000, 759_260_000)])
+def test_device_primes_equal_host_primes(dev, lo, hi):
+    """The device-sieved test-prime list equals the host segmented sieve."""
+    assert np.array_equal(dev.primes(lo, hi), cw.host_primes(lo, hi))

@@ -1,0 +1,77 @@
+"""GPU tests of the request loop (continuous batching): every request's
+results are those of a single-range run, whatever else shared the GPU with
+it, and the HBM budget holds. Integer work: bit-exact comparisons."""
+import sys
+
+import numpy as np
+import pytest
+
+import clwilson_amd as cw
+from clwilson_amd.serve import Server
+from clwilson_amd.workloads import Request
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+RANGES = [(1_000_000, 1_005_000), (5_000_000, 5_004_000), (100_000_000, 100_003_000), (20_000, 40_000),
+          (750_000_000, 750_002_000), (2_000_000_000, 2_000_002_000), (300_000_000, 300_001_000),
+          (1_000_000_000, 1_000_003_000), (7, 100), (60_000_000, 60_005_000), (900_000_000, 900_000_500),
+          (3_000_000, 3_000_100)]
+
+
+def _same(a, b):
+    assert np.array_equal(a.p, b.p)
+    assert np.array_equal(a.quot, b.quot)
+    assert np.array_equal(a.rem, b.rem)
+    assert a.checksum == b.checksum
+    assert (a.resultcount, a.lastp, a.lastw) == (b.resultcount, b.lastp, b.lastw)
+    assert a.results == b.results
+
+
+@pytest.mark.parametrize("streams", [1, 3])
+def test_serving_alone_vs_mix(dev, streams):
+    """Twelve requests arriving over 0.3 s under an HBM budget that admits
+    about three cohorts at a time: every result equals the request run alone
+    (the session path), and the budget is never exceeded."""
+    reqs = [Request(i, 0.025 * i, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
+    budget = 30 << 20
+    srv = Server(hbm_budget=budget, streams=streams, step_seconds=0.01, admit_interval=0.05)
+    try:
+        rep = srv.serve(reqs, duration=100.0, until_done=True)
+    finally:
+        srv.close()
+    say(rep)
+    assert rep["completed"] == len(reqs)
+    assert rep["hbm_peak"] <= budget
+    assert rep["p50_latency_s"] is not None and rep["p99_latency_s"] >= rep["p50_latency_s"]
+    got = {c.rid: c.result for c in srv.completed}
+    for r in reqs:
+        say("request", r.rid, r.pmin, r.pmax)
+        _same(got[r.rid], dev.wilson_range(r.pmin, r.pmax))
+
+
+def test_finish_range_early_matches_full_batch(dev):
+    """A range finished early (its own iterate + check while the batch goes
+    on) equals the same range finished with the whole batch."""
+    ranges = [(1_000_000, 1_002_000), (400_000_000, 400_002_000), (50_000, 60_000)]
+    with cw.Batch(dev, ranges) as b:
+        ref = b.search()
+    with cw.Batch(dev, ranges) as b:
+        maxt = [st.maxtarget for st in b.info]
+        b.run(2, maxt[2] + 1)
+        early = b.finish_range(2)
+        with pytest.raises(cw.ClwError):
+            b.finish_range(1)  # its sieving primes are not all in yet
+        b.run(maxt[2] + 1, maxt[0] + 1)
+        early0 = b.finish_range(0)
+        b.run(maxt[0] + 1, b.maxtarget + 1)
+        b.iterate()
+        b.finalize()
+        _same(early, ref[2])
+        _same(early0, ref[0])
+        _same(b.results(1), ref[1])
+        _same(b.results(2), ref[2])
