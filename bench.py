@@ -48,6 +48,15 @@ METRIC = "primes tested/s at p≈2e13 (batched vs plain), % of int-MAD roofline"
 PMIN, PMAX = 20_000_000_000_000, 20_000_010_000_000
 SPAN = 4370 * 61440  # integers per step (one default segment)
 
+# HBM traffic of k_batch_mul<3,4> on one SPAN segment just below T0 (two
+# launches of `bench.py --profile-run` under rocprofv3 --pmc FETCH_SIZE, then
+# --pmc WRITE_SIZE; KiB per dispatch). The counters are scaled by what they
+# read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,089 KiB
+# for the 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
+# profiles/batch_mul_cfg3_hbm_counters.md.
+BATCH_MUL_TRAFFIC = dict(fetch_kib=1500066.4, write_kib=133789.8, fetch_scale=2.0, write_scale=1.0,
+                         window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
+
 
 def say(*a):
     print("#", *a, file=sys.stderr, flush=True)
@@ -300,6 +309,30 @@ def measure_serving(seconds: float) -> dict:
     return rep
 
 
+def profile_run(cw, args) -> int:
+    """A few kernels to read with rocprofv3: the copy of a known size scales
+    the FETCH_SIZE / WRITE_SIZE counters to bytes; then k_batch_mul on two
+    windows of the production-magnitude range (region boundary T0)."""
+    import torch
+    n = 256 << 20
+    x = torch.full((n,), 7, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    dev = cw.Device(0)
+    run = BatchRunner(cw, dev, args.per_lane, False)
+    top = (run.T[0] + 1) // 30 * 30
+    for k in range(2):
+        run.run(top - (k + 1) * SPAN, top - k * SPAN)
+    cw.lib().clw_finish(dev.ctx)
+    ks = run.stats()
+    run.close()
+    dev.close()
+    print(json.dumps(dict(copy_bytes=n, kernel=run.kernel, launches=ks["mul_launches"],
+                          mul_ms=ks["mul_ms"], record_apps=ks["mul_record_apps"])))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,9 +348,15 @@ def main() -> int:
                     help="windows timed on the one-prime-per-launch path for the batched/plain ratio")
     ap.add_argument("--batch-steps", type=int, default=4,
                     help="windows timed on the 256-range batch (0 = skip it)")
-    ap.add_argument("--no-tune", action="store_true", help="batched engine without the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="256-range batch without the autotuner")
+    ap.add_argument("--tune-single", action="store_true",
+                    help="autotune the one-range batch too (default: 4 test primes per lane, which the "
+                         "autotuner finds within noise of the best at 3 limbs)")
     ap.add_argument("--serve-seconds", type=float, default=45.0,
                     help="wall-clock window of the serving replay (0 = skip it)")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for rocprofv3: a 256 MiB device copy (counter calibration), then two "
+                         "one-segment windows of the [2e13, 2e13+1e7) batch (two k_batch_mul launches)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would search its own "
@@ -326,17 +365,20 @@ def main() -> int:
 
     import clwilson_amd as cw
 
+    if args.profile_run:
+        return profile_run(cw, args)
     dev = cw.Device(0)
     info = dev.info()
     t_setup = time.time()
     if args.engine == "batch":
-        run = BatchRunner(cw, dev, args.per_lane, not args.no_tune)
+        run = BatchRunner(cw, dev, args.per_lane, args.tune_single)
     else:
         run = SessionRunner(cw, dev, args.mode, args.per_lane)
     T, cnt, L = run.T, run.cnt, run.limbs
     regs = regions(T, cnt)
-    # windows just below each region's upper end, disjoint across steps
-    nxt = [r[2] for r in regs]
+    # windows just below each region's upper end (multiples of 30, so each
+    # window is exactly one segment), disjoint across steps
+    nxt = [r[2] // 30 * 30 for r in regs]
 
     def window(ri):
         hi = nxt[ri]
@@ -421,7 +463,10 @@ def main() -> int:
             "peak": peak / 1e12,
             "unit": "Tmad/s",
             "frac": achieved / peak if peak else None,
-            "traffic": None,
+            "traffic": (BATCH_MUL_TRAFFIC["fetch_kib"] * BATCH_MUL_TRAFFIC["fetch_scale"]
+                        + BATCH_MUL_TRAFFIC["write_kib"] * BATCH_MUL_TRAFFIC["write_scale"]) * 1024
+            if run.kernel == "k_batch_mul<3,4>" else None,
+            "traffic_of": BATCH_MUL_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
             "per_unit": f"{prod_per_rec} 30x30-bit partial products per record per test prime",
             "launches": ks["mul_launches"],
             "mean_launch_ms": ks["mul_ms"] / max(1, ks["mul_launches"]),

@@ -105,10 +105,23 @@ CLW_HD void pin_all(uint64_t (&c)[K]) {
   for (int j = 0; j < K; ++j) CLW_PIN1(c[j]);
 }
 
+// Number of low quotient digits m_0..m_{U-1} that montmul_lockstep leaves
+// unmasked (32 bits instead of 30): m_k only has to be right mod 2^30 for the
+// division by 2^30 to be exact, and each mask is one instruction. A 32-bit
+// m_k makes its products m_k n_j < 2^62; U is the largest count for which
+// every column sum (L^2 + L products and the carry) still fits in 64 bits:
+// L = 3: column 2 holds 3 a b + 2 (unmasked m n) + m_2 n_0 + carry
+// < 3 2^60 + 2 2^62 + 2^60 + 2^34 < 2^64.
+template <int L>
+constexpr int unmasked_m() { return (L == 3 || L == 4) ? 2 : 1; }
+
 // K independent Montgomery products r[j] = a[j] * b * R^-1 mod n[j] with a
 // common (wave-uniform) multiplier b, computed column by column in lockstep:
 // consecutive instructions belong to different test primes, so the 64-bit
 // accumulator chains never stall on each other and need no extra adds.
+// Requires a[j] < 2 n[j] and b < n[j] / 4 (every record factor Q <= T^2 <
+// p^2 / 4); then r[j] < 2 n[j] although m may exceed R by ~3 2^(30 U):
+// (a b + m n) / R < n/4 + n (1 + 2^-28) < 2 n.
 template <int L, int K>
 CLW_HD void montmul_lockstep(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, const Res<L> (&n)[K],
                              const uint32_t (&ninv)[K]) {
@@ -139,7 +152,8 @@ CLW_HD void montmul_lockstep(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>&
     if (k < L) {
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        m[j][k] = ((uint32_t)c[j] * ninv[j]) & LMASK;
+        const uint32_t mk = (uint32_t)c[j] * ninv[j];
+        m[j][k] = k < unmasked_m<L>() ? mk : (mk & LMASK);
         c[j] = mad32(m[j][k], n[j].v[0], c[j]) >> LB;
       }
       pin_all<K>(c);

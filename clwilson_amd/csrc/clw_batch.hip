@@ -27,7 +27,7 @@ namespace {
 struct SegHost {  // mirror of BatchSeg (clw_batch_kernels.inc)
   uint64_t lo, hi, base, span, end;
   uint32_t nblk, nbases;
-  uint32_t queue[4];
+  uint32_t queue[3][8];
 };
 struct GroupHost {  // mirror of BatchGroup
   uint64_t T, sqrtT;
@@ -286,7 +286,8 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     }
     size_t tiles = 0;  // at one test prime per lane (the most items any plan has)
     for (const GroupHost& g : bd.groups) tiles += (g.tp_n + 255) / 256;
-    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(16ull * n, 1ull << 24));
+    const uint64_t chunks = std::min<uint32_t>(B->opt.max_chunks ? B->opt.max_chunks : kMaxChunks, kMaxChunks);
+    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(chunks * n, 1ull << 24));
     rc = clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident);
     if (rc == CLW_OK && B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
     if (rc == CLW_OK) rc = clw_tp_table_alloc(ctx, n, (uint32_t)bd.L, &bd.tab);

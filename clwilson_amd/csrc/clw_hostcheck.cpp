@@ -19,6 +19,17 @@ static void montmul_t(const uint64_t a[2], const uint64_t b[2], const uint64_t n
 }
 
 template <int L>
+static void montmul_lockstep_t(const uint64_t a[2], const uint64_t b[2], const uint64_t n[2], uint64_t out[2]) {
+  const u128 A = ((u128)a[1] << 64) | a[0], B = ((u128)b[1] << 64) | b[0], Nv = ((u128)n[1] << 64) | n[0];
+  Res<L> ra[1] = {res_from_u128<L>(A)}, rn[1] = {res_from_u128<L>(Nv)}, r[1];
+  const uint32_t ninv[1] = {neg_inv30(rn[0].v[0])};
+  montmul_lockstep<L, 1>(r, ra, res_from_u128<L>(B), rn, ninv);
+  const u128 R = res_to_u128<L>(r[0]);
+  out[0] = (uint64_t)R;
+  out[1] = (uint64_t)(R >> 64);
+}
+
+template <int L>
 static Ctx<L> make_ctx(uint64_t p) {
   const u128 N = (u128)p * p;
   Ctx<L> c;
@@ -50,6 +61,26 @@ int clwh_montmul(int L, const uint64_t* a, const uint64_t* b, const uint64_t* n,
     case 3: montmul_t<3>(a, b, n, out); return 0;
     case 4: montmul_t<4>(a, b, n, out); return 0;
     case 5: montmul_t<5>(a, b, n, out); return 0;
+  }
+  return -1;
+}
+
+int clwh_montmul_lockstep(int L, const uint64_t* a, const uint64_t* b, const uint64_t* n, uint64_t* out) {
+  switch (L) {
+    case 2: montmul_lockstep_t<2>(a, b, n, out); return 0;
+    case 3: montmul_lockstep_t<3>(a, b, n, out); return 0;
+    case 4: montmul_lockstep_t<4>(a, b, n, out); return 0;
+    case 5: montmul_lockstep_t<5>(a, b, n, out); return 0;
+  }
+  return -1;
+}
+
+int clwh_unmasked_m(int L) {
+  switch (L) {
+    case 2: return unmasked_m<2>();
+    case 3: return unmasked_m<3>();
+    case 4: return unmasked_m<4>();
+    case 5: return unmasked_m<5>();
   }
   return -1;
 }

@@ -185,7 +185,7 @@ class Server:
         for k in sorted(classes):
             take = classes[k]
             b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
-                      segment_apps=int(self.rate * self.step_seconds), host_primes=True)
+                      segment_apps=int(self.rate * self.step_seconds), host_primes=True, max_chunks=8)
             groups = [(st.typeTarget[t_], st.tpcnt[t_]) for st in b.info for t_ in range(3) if st.tpcnt[t_]]
             c = _Cohort(b, -1, take, [st.maxtarget for st in b.info],
                         np.array([g[0] for g in groups], dtype=np.float64),

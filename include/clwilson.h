@@ -231,6 +231,8 @@ typedef struct {
   uint32_t host_primes;    /* batches: test primes from the host sieve on
                               worker threads (no device work while building,
                               e.g. beside busy streams) instead of the device */
+  uint32_t max_chunks;     /* batches: chunks a segment may be split into
+                              (chunk-product slots per test prime; 0 = 64)   */
 } clw_options;
 
 typedef struct {

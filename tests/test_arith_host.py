@@ -23,6 +23,8 @@ def hc():
     u64p = C.POINTER(C.c_uint64)
     L.clwh_montmul.argtypes = [C.c_int, u64p, u64p, u64p, u64p]
     L.clwh_undrift.argtypes = [C.c_int, C.c_uint64, u64p, C.c_uint64, u64p]
+    L.clwh_montmul_lockstep.argtypes = [C.c_int, u64p, u64p, u64p, u64p]
+    L.clwh_unmasked_m.argtypes = [C.c_int]
     L.clwh_neg_inv30.restype = C.c_uint32
     L.clwh_neg_inv30.argtypes = [C.c_uint32]
     L.clwh_legendre.restype = C.c_uint64
@@ -67,6 +69,56 @@ def test_montmul_lazy_bounds(hc, L):
         r = out[0] | (out[1] << 64)
         assert r < 3 * N and r % N == a * b2 * pow(R, -1, N) % N
         n_ok += 1
+
+
+def _column_bound(L, U):
+    """Largest 64-bit column sum of montmul_lockstep: limbs of a, b, n below
+    2^30, quotient digits m_k below 2^32 for k < U (unmasked), else 2^30."""
+    A = 2**30 - 1
+    M = [2**32 - 1 if k < U else A for k in range(L)]
+    carry = worst = 0
+    for k in range(2 * L - 1):
+        s = carry + sum(A * A for i in range(L) if 0 <= k - i < L)
+        s += sum(M[i] * A for i in range(L) if i < k and 0 <= k - i < L)
+        if k < L:
+            s += M[k] * A
+        worst = max(worst, s)
+        carry = s >> 30
+    return worst
+
+
+@pytest.mark.parametrize("L", [2, 3, 4, 5])
+def test_lockstep_unmasked_digits_fit(hc, L):
+    """The unmasked quotient digits keep every column below 2^64, the top
+    digit stays masked (so m < R (1 + 2^-26)), and for 3 and 4 limbs one
+    more unmasked digit would overflow a column."""
+    U = hc.clwh_unmasked_m(L)
+    assert 1 <= U <= L - 1
+    assert _column_bound(L, U) < 2**64
+    if L in (3, 4):
+        assert U == L - 1 or _column_bound(L, U + 1) >= 2**64
+
+
+@pytest.mark.parametrize("L", [2, 3, 4, 5])
+def test_montmul_lockstep_bounds(hc, L):
+    """The lockstep product (unmasked low digits) is a b R^-1 mod N and stays
+    below 2N for a < 2N and b < N/4 (record factors Q <= T^2 < p^2/4), for
+    random primes of the band and the largest prime of the band, with
+    extreme operands."""
+    R = 2 ** (30 * L)
+    rng = random.Random(100 + L)
+    pmax = {2: 759250124, 3: 24879108095803, 4: 815238614083298888, 5: 2**62}[L]
+    primes = [sympy.prevprime(pmax)] + [sympy.nextprime(rng.randrange(pmax // 4, pmax)) for _ in range(40)]
+    out = (C.c_uint64 * 2)()
+    for p in primes:
+        N = p * p
+        assert 2 * N < R
+        cases = [(2 * N - 1, N // 4 - 1), (2 * N - 1, 1), (0, N // 4 - 1)]
+        cases += [(rng.randrange(2 * N), rng.randrange(N // 4)) for _ in range(60)]
+        for a, b in cases:
+            hc.clwh_montmul_lockstep(L, _arr(a), _arr(b), _arr(N), out)
+            r = out[0] | (out[1] << 64)
+            assert r < 2 * N and r % N == a * b * pow(R, -1, N) % N, (L, p, a, b)
 
 
 @pytest.mark.parametrize("L", [2, 3, 4, 5])
