@@ -186,6 +186,8 @@ def lib():
         "clw_batch_wait": (i32, [vp]),
         "clw_batch_query": (i32, [vp]),
         "clw_batch_bind": (i32, [vp, vp]),
+        "clw_batch_ckpt_write": (i32, [vp, C.c_char_p]),
+        "clw_batch_ckpt_read": (i32, [vp, C.c_char_p, C.c_char_p, P(i32), P(i32)]),
         "clw_batch_covered": (u64, [vp]),
         "clw_batch_device_bytes": (u64, [vp]),
         "clw_batch_finish_range": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
@@ -460,6 +462,18 @@ class Batch:
 
     def wait(self) -> None:
         check(lib().clw_batch_wait(self.h), "batch_wait")
+
+    def checkpoint(self, path: str) -> None:
+        """Write every range's running state and the covered prefix."""
+        check(lib().clw_batch_ckpt_write(self.h, path.encode()), "batch_ckpt_write")
+
+    def resume(self, path_a: str, path_b: str) -> tuple[int, int]:
+        """(result, which): 1 if a valid checkpoint was loaded (continue from
+        covered()), else 0; which = 0 for path_a, 1 for path_b."""
+        which, result = C.c_int(-1), C.c_int(0)
+        check(lib().clw_batch_ckpt_read(self.h, path_a.encode(), path_b.encode(), C.byref(which),
+                                        C.byref(result)), "batch_ckpt_read")
+        return result.value, which.value
 
     def bind(self, dev: "Device") -> None:
         """Run this batch on another device context (stream) from now on."""

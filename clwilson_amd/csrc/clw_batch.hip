@@ -892,3 +892,170 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
   B->graph = nullptr;
   return CLW_OK;
 }
+
+// ---- checkpoint / resume of a batch ----------------------------------------
+//
+// The reference's A/B checkpoint (cl_wilson.cpp:110-319: a header with the
+// range, progress and a checksum, then the residues; two files written in
+// turn, the valid one with more progress read back) extended to a batch:
+//   header  "CLWBATCH", version, ranges, covered (sieving primes [2, covered)
+//           applied to every range), applied count, state_sum
+//   ranges  {pmin, pmax, tpcount, finished}
+//   data    per range in increasing p: canonical residues (lo, hi), or for a
+//           range already finished on its own, its (quot, rem)
+// state_sum = the additive sum (mod 2^64) of every other 64-bit word.
+
+namespace {
+
+const char kBatchMagic[8] = {'C', 'L', 'W', 'B', 'A', 'T', 'C', 'H'};
+const uint64_t kBatchCkptVersion = 1;
+
+struct CkptHead {
+  char magic[8];
+  uint64_t version, nranges, covered, applied, state_sum;
+};
+struct CkptRange {
+  uint64_t pmin, pmax, tpcount, finished;
+};
+
+uint64_t sum_words(const void* p, size_t bytes) {
+  const uint64_t* w = static_cast<const uint64_t*>(p);
+  uint64_t s = 0;
+  for (size_t i = 0; i < bytes / 8; ++i) s += w[i];
+  return s;
+}
+
+}  // namespace
+
+extern "C" int clw_batch_ckpt_write(clw_batch* B, const char* path) {
+  if (!B || !path) return clw_fail(CLW_ERR_ARG, "clw_batch_ckpt_write: null argument");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_write with a window in flight");
+  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_write after clw_batch_iterate");
+  std::vector<CkptRange> rr(B->ranges.size());
+  std::vector<uint64_t> data;
+  std::vector<uint64_t> acc_lo[6], acc_hi[6];
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
+    if (!bd.L) continue;
+    acc_lo[band].resize(bd.tab.count);
+    acc_hi[band].resize(bd.tab.count);
+    CK(clw_d2h(B->ctx, acc_lo[band].data(), bd.tab.acc_lo, (size_t)bd.tab.count * 8, 0));
+    CK(clw_d2h(B->ctx, acc_hi[band].data(), bd.tab.acc_hi, (size_t)bd.tab.count * 8, 1));
+  }
+  for (size_t r = 0; r < B->ranges.size(); ++r) {
+    const RangeHost& R = B->ranges[r];
+    rr[r] = {R.pmin, R.pmax, (uint64_t)R.tp.size(), R.finished ? 1ull : 0ull};
+    for (size_t k = 0; k < R.tp.size(); ++k) {
+      const uint32_t i = R.pos[k];
+      if (R.finished) {
+        data.push_back(R.quot[i - R.band_off]);
+        data.push_back(R.rem[i - R.band_off]);
+      } else {
+        data.push_back(acc_lo[R.band][i]);
+        data.push_back(acc_hi[R.band][i]);
+      }
+    }
+  }
+  CkptHead h{};
+  memcpy(h.magic, kBatchMagic, 8);
+  h.version = kBatchCkptVersion;
+  h.nranges = B->ranges.size();
+  h.covered = B->next_q;
+  h.applied = B->applied;
+  h.state_sum = h.version + h.nranges + h.covered + h.applied + sum_words(rr.data(), rr.size() * sizeof(CkptRange)) +
+                sum_words(data.data(), data.size() * 8);
+  FILE* f = fopen(path, "wb");
+  if (!f) return clw_fail(CLW_ERR_IO, "cannot open %s", path);
+  bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+  ok = ok && fwrite(rr.data(), sizeof(CkptRange), rr.size(), f) == rr.size();
+  ok = ok && (data.empty() || fwrite(data.data(), 8, data.size(), f) == data.size());
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return clw_fail(CLW_ERR_IO, "cannot write batch checkpoint %s", path);
+  return CLW_OK;
+}
+
+// Reads one checkpoint file; 1 if it is valid for this batch (head and data
+// filled), 0 if missing, unreadable, for other ranges or failing its sum.
+static int read_batch_file(const clw_batch* B, const char* path, CkptHead* h, std::vector<CkptRange>* rr,
+                           std::vector<uint64_t>* data) {
+  if (!path) return 0;
+  FILE* f = fopen(path, "rb");
+  if (!f) return 0;
+  bool ok = fread(h, sizeof(*h), 1, f) == 1 && !memcmp(h->magic, kBatchMagic, 8) &&
+            h->version == kBatchCkptVersion && h->nranges == B->ranges.size();
+  uint64_t words = 0;
+  if (ok) {
+    rr->resize(h->nranges);
+    ok = fread(rr->data(), sizeof(CkptRange), rr->size(), f) == rr->size();
+    for (size_t r = 0; ok && r < rr->size(); ++r) {
+      const RangeHost& R = B->ranges[r];
+      const CkptRange& c = (*rr)[r];
+      ok = c.pmin == R.pmin && c.pmax == R.pmax && c.tpcount == R.tp.size() && c.finished <= 1;
+      words += 2 * c.tpcount;
+    }
+  }
+  if (ok) {
+    data->resize(words);
+    ok = words == 0 || fread(data->data(), 8, words, f) == words;
+  }
+  fclose(f);
+  if (!ok) return 0;
+  const uint64_t s = h->version + h->nranges + h->covered + h->applied +
+                     sum_words(rr->data(), rr->size() * sizeof(CkptRange)) + sum_words(data->data(), words * 8);
+  return s == h->state_sum ? 1 : 0;
+}
+
+extern "C" int clw_batch_ckpt_read(clw_batch* B, const char* path_a, const char* path_b, int* which,
+                                   int* result) {
+  if (!B || !which || !result) return clw_fail(CLW_ERR_ARG, "clw_batch_ckpt_read: null argument");
+  if (B->pending.active || B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_read: batch in use");
+  *which = -1;
+  *result = 0;
+  CkptHead ha{}, hb{};
+  std::vector<CkptRange> ra, rb;
+  std::vector<uint64_t> da, db;
+  const int va = read_batch_file(B, path_a, &ha, &ra, &da);
+  const int vb = read_batch_file(B, path_b, &hb, &rb, &db);
+  if (!va && !vb) return CLW_OK;
+  const bool use_b = vb && (!va || hb.covered > ha.covered);
+  const CkptHead& h = use_b ? hb : ha;
+  const std::vector<CkptRange>& rr = use_b ? rb : ra;
+  const std::vector<uint64_t>& d = use_b ? db : da;
+  std::vector<uint64_t> acc_lo[6], acc_hi[6];
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
+    if (!bd.L) continue;
+    acc_lo[band].assign(bd.tab.count, 0);
+    acc_hi[band].assign(bd.tab.count, 0);
+  }
+  size_t w = 0;
+  for (size_t r = 0; r < B->ranges.size(); ++r) {
+    RangeHost& R = B->ranges[r];
+    R.finished = rr[r].finished != 0;
+    if (R.finished) {
+      R.quot.assign(R.tp.size(), 0);
+      R.rem.assign(R.tp.size(), 0);
+    }
+    for (size_t k = 0; k < R.tp.size(); ++k, w += 2) {
+      const uint32_t i = R.pos[k];
+      if (R.finished) {
+        R.quot[i - R.band_off] = d[w];
+        R.rem[i - R.band_off] = d[w + 1];
+      } else {
+        acc_lo[R.band][i] = d[w];
+        acc_hi[R.band][i] = d[w + 1];
+      }
+    }
+  }
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
+    if (!bd.L) continue;
+    CK(clw_h2d(B->ctx, bd.tab.acc_lo, acc_lo[band].data(), (size_t)bd.tab.count * 8, 0));
+    CK(clw_h2d(B->ctx, bd.tab.acc_hi, acc_hi[band].data(), (size_t)bd.tab.count * 8, 1));
+  }
+  B->next_q = h.covered;
+  B->applied = h.applied;
+  *which = use_b ? 1 : 0;
+  *result = 1;
+  return CLW_OK;
+}

@@ -27,7 +27,9 @@ stream and are served together:
 """
 from __future__ import annotations
 
+import json
 import math
+import os
 import threading
 import time
 from dataclasses import dataclass, field
@@ -184,14 +186,7 @@ class Server:
         n = 0
         for k in sorted(classes):
             take = classes[k]
-            b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
-                      segment_apps=int(self.rate * self.step_seconds), host_primes=True, max_chunks=8)
-            groups = [(st.typeTarget[t_], st.tpcnt[t_]) for st in b.info for t_ in range(3) if st.tpcnt[t_]]
-            c = _Cohort(b, -1, take, [st.maxtarget for st in b.info],
-                        np.array([g[0] for g in groups], dtype=np.float64),
-                        np.array([g[1] for g in groups], dtype=np.int64), self.now())
-            c.bytes = b.device_bytes()
-            c.rate = self.rate / 8  # factor applications/s, refined by every window it runs
+            c = self._build(take)
             n += len(take)
             with self.lock:
                 self.ready.append(c)
@@ -200,6 +195,17 @@ class Server:
                 self.peak_used = max(self.peak_used, self.used)
         self.admit_seconds += time.perf_counter() - t
         return n
+
+    def _build(self, take: list[Request]) -> _Cohort:
+        """A cohort of requests: their batch built on the admission stream."""
+        b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
+                  segment_apps=int(self.rate * self.step_seconds), host_primes=True, max_chunks=8)
+        groups = [(st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t]]
+        c = _Cohort(b, -1, take, [st.maxtarget for st in b.info], np.array([g[0] for g in groups], dtype=np.float64),
+                    np.array([g[1] for g in groups], dtype=np.int64), self.now())
+        c.bytes = b.device_bytes()
+        c.rate = self.rate / 8  # factor applications/s, refined by every window it runs
+        return c
 
     def _admission_loop(self) -> None:
         try:
@@ -307,6 +313,52 @@ class Server:
         for slot, c in enumerate(self.running):
             if c is not None:
                 self._complete(slot)
+
+    # -- snapshot / restore -------------------------------------------------------
+    def snapshot(self, directory: str) -> None:
+        """Checkpoint the server: every cohort's batch (clw_batch_ckpt_write:
+        residues, covered prefix, ranges finished on their own) and a
+        manifest of cohorts, their requests and the queue. Windows in flight
+        are waited for first; admission must be stopped."""
+        if self._thread is not None:
+            raise RuntimeError("stop admission before a snapshot")
+        self.drain()
+        self._bind_ready()
+        os.makedirs(directory, exist_ok=True)
+        cohorts = []
+        for i, c in enumerate(self.cohorts):
+            path = os.path.join(directory, f"cohort{i}.ckpt")
+            c.batch.checkpoint(path)
+            cohorts.append(dict(file=os.path.basename(path), done=sorted(c.done), admitted=c.admitted,
+                                requests=[[r.rid, r.arrival, r.pmin, r.pmax] for r in c.reqs]))
+        with self.lock:
+            queue = [[r.rid, r.arrival, r.pmin, r.pmax] for r in self.queue]
+        with open(os.path.join(directory, "manifest.json"), "w") as f:
+            json.dump(dict(cohorts=cohorts, queue=queue, completed=[c.rid for c in self.completed]), f)
+
+    def restore(self, directory: str) -> list[int]:
+        """Rebuild the cohorts of a snapshot (same ranges, residues read back)
+        and its queue; returns the ids of requests completed before it."""
+        with open(os.path.join(directory, "manifest.json")) as f:
+            m = json.load(f)
+        for entry in m["cohorts"]:
+            take = [Request(*r) for r in entry["requests"]]
+            c = self._build(take)
+            path = os.path.join(directory, entry["file"])
+            result, _ = c.batch.resume(path, path)
+            if result != 1:
+                raise RuntimeError(f"snapshot file {path} is not valid for its cohort")
+            c.done = set(entry["done"])
+            c.admitted = entry["admitted"]
+            c.work = c.work_to_next()
+            with self.lock:
+                self.ready.append(c)
+                self.used += c.bytes
+                self.peak_used = max(self.peak_used, self.used)
+        with self.lock:
+            self.queue.extend(Request(*r) for r in m["queue"])
+        self._bind_ready()
+        return m["completed"]
 
     def idle(self) -> bool:
         with self.lock:

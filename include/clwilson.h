@@ -361,6 +361,15 @@ typedef struct {
 } clw_band_tune;
 int clw_batch_autotune(clw_batch* b, uint64_t q, const char* cache_path, clw_band_tune* report,
                        uint32_t report_cap);
+/* Checkpoint of a batch (the A/B scheme of cl_wilson.cpp:110-319 extended
+ * to many ranges): every range's canonical running residues (or, for a range
+ * already finished on its own, its results) and the covered prefix of
+ * sieving primes, with an additive checksum. Write alternately to two files;
+ * read picks the valid file (right ranges, right sum) that covers more.
+ * *result = 0 (no valid file: start from scratch) or 1 (resumed; continue
+ * with windows from clw_batch_covered(); *which = 0 for a, 1 for b). */
+int clw_batch_ckpt_write(clw_batch* b, const char* path);
+int clw_batch_ckpt_read(clw_batch* b, const char* path_a, const char* path_b, int* which, int* result);
 /* segments, sieve_integers, record/prime applications (device counters);
  * sieve_ms, mul_ms, reduce_ms from HIP events when the batch launches its
  * steps eagerly (opt.graph = 0; 0 under graph replay); side_ms = host wall

@@ -188,3 +188,44 @@ def test_batch_host_prime_source(dev):
     with cw.Batch(dev, MIX[2:8], host_primes=True) as a, cw.Batch(dev, MIX[2:8]) as b:
         for x, y in zip(a.search(), b.search()):
             _same(x, y)
+
+
+def test_batch_checkpoint_resume(dev, tmp_path):
+    """A batch checkpointed part-way (A, then B), rebuilt and resumed from
+    the newer valid file, ends with the uninterrupted results; a corrupt B
+    falls back to A; a range finished before the checkpoint stays finished."""
+    ranges = MIX[2:8]
+    with cw.Batch(dev, ranges) as b:
+        ref = b.search()
+    a, bp = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    with cw.Batch(dev, ranges) as b:
+        b.run(2, 10_000_000)
+        b.checkpoint(a)
+        m2 = b.info[2].maxtarget  # (86e6, 86.01e6): target ~4.3e7
+        assert m2 > 10_000_000
+        b.run(10_000_000, m2 + 1)
+        early = [b.finish_range(0), b.finish_range(2)]
+        b.checkpoint(bp)
+    _same(early[0], ref[0])
+    _same(early[1], ref[2])
+    with cw.Batch(dev, ranges) as b:
+        assert b.resume(a, bp) == (1, 1)
+        assert b.covered() == m2 + 1
+        b.run(b.covered(), b.maxtarget + 1)
+        b.iterate()
+        b.finalize()
+        for r in range(len(ranges)):
+            _same(b.results(r), ref[r])
+    raw = bytearray(open(bp, "rb").read())
+    raw[-3] ^= 0x40
+    open(bp, "wb").write(bytes(raw))
+    with cw.Batch(dev, ranges) as b:
+        assert b.resume(a, bp) == (1, 0)
+        assert b.covered() == 10_000_000
+        b.run(b.covered(), b.maxtarget + 1)
+        b.iterate()
+        b.finalize()
+        for r in range(len(ranges)):
+            _same(b.results(r), ref[r])
+    with cw.Batch(dev, ranges[:3]) as b:  # other ranges: no valid file
+        assert b.resume(a, bp)[0] == 0

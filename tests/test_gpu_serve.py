@@ -75,3 +75,35 @@ def test_finish_range_early_matches_full_batch(dev):
         _same(early0, ref[0])
         _same(b.results(1), ref[1])
         _same(b.results(2), ref[2])
+
+
+def test_server_snapshot_restore(dev, tmp_path):
+    """A server stopped part-way and snapshotted, then restored into a new
+    server, completes every request with the single-range results."""
+    reqs = [Request(i, 0.0, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
+    srv = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.002)
+    try:
+        for r in reqs:
+            srv.submit(r)
+        srv.admit()
+        for _ in range(6):
+            srv.poll()
+        srv.snapshot(str(tmp_path))
+        before = {c.rid: c.result for c in srv.completed}
+    finally:
+        srv.close()
+    say("completed before the snapshot:", sorted(before))
+    srv2 = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.01)
+    try:
+        done = srv2.restore(str(tmp_path))
+        assert sorted(done) == sorted(before)
+        while not srv2.idle():
+            if not srv2.poll():
+                pass
+        after = {c.rid: c.result for c in srv2.completed}
+    finally:
+        srv2.close()
+    got = {**before, **after}
+    assert sorted(got) == [r.rid for r in reqs]
+    for r in reqs:
+        _same(got[r.rid], dev.wilson_range(r.pmin, r.pmax))
