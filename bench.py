@@ -51,10 +51,10 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # HBM traffic of k_batch_mul<3,4> on one SPAN segment just below T0 (two
 # launches of `bench.py --profile-run` under rocprofv3 --pmc FETCH_SIZE, then
 # --pmc WRITE_SIZE; KiB per dispatch). The counters are scaled by what they
-# read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,089 KiB
+# read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB
 # for the 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
 # profiles/batch_mul_cfg3_hbm_counters.md.
-BATCH_MUL_TRAFFIC = dict(fetch_kib=1500066.4, write_kib=133789.8, fetch_scale=2.0, write_scale=1.0,
+BATCH_MUL_TRAFFIC = dict(fetch_kib=1234392.2, write_kib=133782.9, fetch_scale=2.0, write_scale=1.0,
                          window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
 
 
