@@ -95,7 +95,8 @@ class WorkStatus(C.Structure):
 class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
                 ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32),
-                ("segment_apps", C.c_uint64), ("host_primes", C.c_uint32), ("max_chunks", C.c_uint32)]
+                ("segment_apps", C.c_uint64), ("host_primes", C.c_uint32), ("max_chunks", C.c_uint32),
+                ("external_scratch", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -186,6 +187,7 @@ def lib():
         "clw_batch_wait": (i32, [vp]),
         "clw_batch_query": (i32, [vp]),
         "clw_batch_bind": (i32, [vp, vp]),
+        "clw_batch_set_scratch": (i32, [vp, vp, vp, u64]),
         "clw_batch_ckpt_write": (i32, [vp, C.c_char_p]),
         "clw_batch_ckpt_read": (i32, [vp, C.c_char_p, C.c_char_p, P(i32), P(i32)]),
         "clw_batch_covered": (u64, [vp]),
@@ -414,13 +416,31 @@ class Session:
         return res
 
 
+class Scratch:
+    """Device buffers for chunk products (2 x cap 64-bit slots) shared by
+    batches that run one after another on one stream."""
+
+    def __init__(self, dev: "Device", cap: int):
+        self.dev, self.cap = dev, cap
+        lo, hi = C.c_void_p(), C.c_void_p()
+        check(lib().clw_alloc(dev.ctx, cap * 8, C.byref(lo)), "scratch alloc")
+        check(lib().clw_alloc(dev.ctx, cap * 8, C.byref(hi)), "scratch alloc")
+        self.lo, self.hi = lo.value, hi.value
+
+    def close(self) -> None:
+        if self.lo:
+            lib().clw_free(self.dev.ctx, self.lo)
+            lib().clw_free(self.dev.ctx, self.hi)
+            self.lo = self.hi = None
+
+
 class Batch:
     """Many ranges searched together over one shared sieve (clw_batch_*).
     Each range's results equal clw_wilson_range's on it, bit for bit."""
 
     def __init__(self, dev: "Device", ranges: list[tuple[int, int]], segment_span: int = 0,
                  target_waves: int = 0, per_lane: int = 0, graph: bool = True, segment_apps: int = 0,
-                 host_primes: bool = False, max_chunks: int = 0):
+                 host_primes: bool = False, max_chunks: int = 0, external_scratch: bool = False):
         if not ranges:
             raise ValueError("a batch needs at least one range")
         self.dev = dev
@@ -429,7 +449,7 @@ class Batch:
         lo = np.array([a for a, _ in self.ranges], dtype=np.uint64)
         hi = np.array([b for _, b in self.ranges], dtype=np.uint64)
         opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0, segment_apps,
-                      1 if host_primes else 0, max_chunks)
+                      1 if host_primes else 0, max_chunks, 1 if external_scratch else 0)
         check(lib().clw_batch_create(dev.ctx, lo.ctypes.data, hi.ctypes.data, len(self.ranges), C.byref(opt),
                                      C.byref(self.h)), "batch_create")
         self.info = []
@@ -474,6 +494,10 @@ class Batch:
         check(lib().clw_batch_ckpt_read(self.h, path_a.encode(), path_b.encode(), C.byref(which),
                                         C.byref(result)), "batch_ckpt_read")
         return result.value, which.value
+
+    def set_scratch(self, scratch: "Scratch") -> None:
+        """Keep chunk products in a shared device scratch (see Scratch)."""
+        check(lib().clw_batch_set_scratch(self.h, scratch.lo, scratch.hi, scratch.cap), "batch_set_scratch")
 
     def bind(self, dev: "Device") -> None:
         """Run this batch on another device context (stream) from now on."""

@@ -68,6 +68,7 @@ struct Band {
   PlanHost* plan_d = nullptr;
   PlanHost plan{};
   uint64_t part_cap = 0;   // chunk-product slots
+  bool part_own = true;    // part_lo/hi allocated by the batch (else a caller's scratch)
   uint32_t resident = 0;   // persistent grid of the product kernel
   uint32_t items_per_block = kItemsPerBlock;
   uint64_t active_tp = 0;  // test primes of the groups active in this window
@@ -158,8 +159,12 @@ struct clw_batch {
     for (Band& b : bands) {
       if (!b.L) continue;
       clw_tp_table_free(ctx, &b.tab);
-      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.part_lo, b.part_hi, b.plan_d};
+      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.plan_d};
       for (void* x : ptrs) clw_free(ctx, x);
+      if (b.part_own) {
+        clw_free(ctx, b.part_lo);
+        clw_free(ctx, b.part_hi);
+      }
     }
     void* ptrs[] = {rec, nrec, flags, totals, seg_d};
     for (void* x : ptrs) clw_free(ctx, x);
@@ -302,8 +307,13 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     if (rc == CLW_OK) rc = balloc(sizeof(PlanHost), (void**)&bd.plan_d);
     if (rc == CLW_OK) rc = balloc((size_t)n * 8, (void**)&bd.from_d);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.from_d, bd.from.data(), (size_t)n * 8, 1);
-    if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_lo);
-    if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_hi);
+    if (B->opt.external_scratch) {  // chunk products in a caller's buffer (clw_batch_set_scratch)
+      bd.part_cap = 0;
+      bd.part_own = false;
+    } else {
+      if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_lo);
+      if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_hi);
+    }
   }
   if (rc == CLW_OK) rc = clw_finish(ctx);
   if (rc != CLW_OK) {
@@ -345,6 +355,40 @@ extern "C" int clw_batch_info(const clw_batch* B, uint32_t r, clw_range_stats* s
   }
   st->maxtarget = std::max(R.T[0], std::max(R.T[1], R.T[2]));
   st->limbs = (uint32_t)R.band;
+  return CLW_OK;
+}
+
+// Chunk products in a caller's scratch of `cap` slots (lo and hi words),
+// shared by batches that run one after another on one stream; split among
+// the batch's limb bands by their test primes.
+extern "C" int clw_batch_set_scratch(clw_batch* B, uint64_t* part_lo, uint64_t* part_hi, uint64_t cap) {
+  if (!B || !part_lo || !part_hi) return clw_fail(CLW_ERR_ARG, "clw_batch_set_scratch: null argument");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_set_scratch with a window in flight");
+  uint64_t total = 0;
+  for (const Band& bd : B->bands) total += bd.L ? bd.tab.count : 0;
+  if (cap < total) return clw_fail(CLW_ERR_ARG, "scratch of %llu slots < %llu test primes", (unsigned long long)cap,
+                                   (unsigned long long)total);
+  uint64_t off = 0;
+  for (Band& bd : B->bands) {
+    if (!bd.L) continue;
+    if (bd.part_own) {
+      clw_free(B->ctx, bd.part_lo);
+      clw_free(B->ctx, bd.part_hi);
+      B->bytes -= 16 * bd.part_cap;
+      bd.part_own = false;
+    }
+    const uint64_t share = (uint64_t)((double)cap * bd.tab.count / total);
+    bd.part_cap = std::max<uint64_t>(bd.tab.count, std::min<uint64_t>(share, cap - off));
+    bd.part_lo = part_lo + off;
+    bd.part_hi = part_hi + off;
+    off += bd.part_cap;
+  }
+  if (off > cap) return clw_fail(CLW_ERR_ARG, "scratch too small for the bands of this batch");
+  // kernel arguments changed: capture the segment step again
+  if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  B->graph_exec = nullptr;
+  B->graph = nullptr;
   return CLW_OK;
 }
 
@@ -443,6 +487,9 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   if (qlo < 2) qlo = 2;
   if (qhi <= qlo) return CLW_OK;
   const uint64_t lo = std::max<uint64_t>(qlo, 7);
+  for (const Band& bd : B->bands)
+    if (bd.L && bd.part_cap < bd.tab.count)
+      return clw_fail(CLW_ERR_STATE, "batch without chunk-product slots (clw_batch_set_scratch)");
   uint64_t active = 0;
   CK(plan_window(B, qlo, &active));
   uint64_t span_blocks = B->max_blocks - 2;

@@ -16,7 +16,9 @@ stream and are served together:
   on `streams` HIP streams (one device context each): the event loop keeps a
   window of sieving primes in flight on every stream, so the kernels of
   different cohorts overlap on the GPU, and starts the next window on a
-  stream as soon as its last one is done (no round barrier).
+  stream as soon as its last one is done (no round barrier). Stream 0 serves
+  the cohorts large enough to fill the GPU by themselves; the others serve
+  the smaller ones, whose kernels fill the gaps beside them.
 * Scheduling. Shortest work to the next completion first: the cohorts whose
   next range needs the fewest factor applications to finish go first, and a
   window stops at that range's largest factorial target, so a range finishes
@@ -36,7 +38,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Batch, Device, RangeResult, host_primes
+from . import Batch, Device, RangeResult, Scratch, host_primes
 from .workloads import Request, pi_approx
 
 BLOCK_SPAN = 61440
@@ -111,6 +113,9 @@ class Server:
                  admit_interval: float = 0.5):
         self.devices = [Device(device) for _ in range(streams)]
         self.admit_dev = Device(device)
+        # chunk products of every cohort on a stream live in that stream's
+        # scratch (windows on one stream run one after another)
+        self.scratch = [Scratch(d, self.SCRATCH_SLOTS) for d in self.devices]
         self.budget = hbm_budget
         self.step_seconds = step_seconds
         self.rate = apps_per_second
@@ -122,7 +127,7 @@ class Server:
         self.ready: list[_Cohort] = []       # built, not yet bound to a stream
         self.cohorts: list[_Cohort] = []
         self.running: list[_Cohort | None] = [None] * streams
-        self.used = 0                        # device bytes of built cohorts
+        self.used = len(self.devices) * self.SCRATCH_SLOTS * 16  # scratch + built cohorts (device bytes)
         self.reserved = 0                    # estimated bytes of cohorts being built
         self.peak_used = 0
         self.completed: list[Completion] = []
@@ -135,12 +140,18 @@ class Server:
         self._error: BaseException | None = None
         self.t0 = time.perf_counter()
 
+    SCRATCH_SLOTS = 1 << 23    # chunk-product slots per stream (2 x 64 MiB)
+    COHORT_PRIMES = 1 << 22    # estimated test primes per cohort at most
+    FILL_PRIMES = 1 << 18      # test primes that fill the GPU in one window
+
     def close(self) -> None:
         self.stop_admission()
         for c in self.cohorts + self.ready:
             c.batch.wait()
             c.batch.close()
         self.cohorts, self.ready = [], []
+        for sc in self.scratch:
+            sc.close()
         for d in self.devices + [self.admit_dev]:
             d.close()
 
@@ -153,11 +164,15 @@ class Server:
 
     # -- admission (its own thread) ---------------------------------------------
     @staticmethod
-    def estimate_bytes(req: Request) -> int:
-        """Upper estimate of a request's device footprint: ~1.1 width/ln p
-        test primes x (table 88 B + plan and chunk-product slots)."""
-        n = 1.1 * (req.pmax - req.pmin) / math.log(max(req.pmin, 3)) + 64
-        return int(n * 400)
+    def estimate_primes(req: Request) -> float:
+        """~1.1 width / ln p test primes (an upper estimate)."""
+        return 1.1 * (req.pmax - req.pmin) / math.log(max(req.pmin, 3)) + 64
+
+    @classmethod
+    def estimate_bytes(cls, req: Request) -> int:
+        """Upper estimate of a request's device footprint: its test primes x
+        (table 88 B + group index, start target and plan ~32 B)."""
+        return int(cls.estimate_primes(req) * 120)
 
     @staticmethod
     def magnitude_class(req: Request) -> int:
@@ -169,18 +184,21 @@ class Server:
         fixed = (self.segment_span // BLOCK_SPAN + 2) * 4096 * 32 + (1 << 20)
         classes: dict[int, list[Request]] = {}
         est: dict[int, int] = {}
+        primes: dict[int, float] = {}
         with self.lock:
             while self.queue:
                 r = self.queue[0]
                 k = self.magnitude_class(r)
                 grp = classes.get(k)
-                if grp is not None and len(grp) >= self.max_cohort:
+                if grp is not None and (len(grp) >= self.max_cohort or
+                                        primes[k] + self.estimate_primes(r) > self.COHORT_PRIMES):
                     break
                 e = self.estimate_bytes(r) + (0 if grp else fixed)
                 if self.used + self.reserved + e > self.budget:
                     break
                 classes.setdefault(k, []).append(self.queue.pop(0))
                 est[k] = est.get(k, 0) + e
+                primes[k] = primes.get(k, 0.0) + self.estimate_primes(r)
                 self.reserved += e
         t = time.perf_counter()
         n = 0
@@ -199,7 +217,7 @@ class Server:
     def _build(self, take: list[Request]) -> _Cohort:
         """A cohort of requests: their batch built on the admission stream."""
         b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
-                  segment_apps=int(self.rate * self.step_seconds), host_primes=True, max_chunks=8)
+                  segment_apps=int(self.rate * self.step_seconds), host_primes=True, external_scratch=True)
         groups = [(st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t]]
         c = _Cohort(b, -1, take, [st.maxtarget for st in b.info], np.array([g[0] for g in groups], dtype=np.float64),
                     np.array([g[1] for g in groups], dtype=np.int64), self.now())
@@ -244,10 +262,15 @@ class Server:
         with self.lock:
             ready, self.ready = self.ready, []
         for c in ready:
-            # the stream with the least queued work
-            load = [sum(1 for x in self.cohorts if x.slot == s) for s in range(len(self.devices))]
-            c.slot = min(range(len(self.devices)), key=lambda s: load[s])
+            # stream 0 takes the cohorts with enough test primes to fill the
+            # GPU on their own (throughput), the others the smaller ones
+            # (latency), each the least loaded of its kind
+            n = int(c.n.sum())
+            kinds = [0] if (n >= self.FILL_PRIMES or len(self.devices) == 1) else list(range(1, len(self.devices)))
+            load = {s: sum(1 for x in self.cohorts if x.slot == s) for s in kinds}
+            c.slot = min(kinds, key=lambda s: load[s])
             c.batch.bind(self.devices[c.slot])
+            c.batch.set_scratch(self.scratch[c.slot])
             self.cohorts.append(c)
 
     def _launch(self, slot: int) -> bool:

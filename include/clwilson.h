@@ -233,6 +233,8 @@ typedef struct {
                               e.g. beside busy streams) instead of the device */
   uint32_t max_chunks;     /* batches: chunks a segment may be split into
                               (chunk-product slots per test prime; 0 = 64)   */
+  uint32_t external_scratch; /* batches: allocate no chunk-product slots; the
+                              caller provides them (clw_batch_set_scratch)   */
 } clw_options;
 
 typedef struct {
@@ -306,6 +308,10 @@ int clw_batch_destroy(clw_batch* b);
 /* Moves the batch to another context (stream) of the same device; no window
  * may be in flight. Lets one thread build batches while others run. */
 int clw_batch_bind(clw_batch* b, clw_ctx* ctx);
+/* Chunk products in caller-owned device buffers of cap 64-bit slots each
+ * (at least the batch's test primes): batches that run one after another on
+ * one stream can share one scratch instead of holding their own. */
+int clw_batch_set_scratch(clw_batch* b, uint64_t* part_lo, uint64_t* part_hi, uint64_t cap);
 uint32_t clw_batch_size(const clw_batch* b);
 /* Counts and targets of range r (as clw_session_info). */
 int clw_batch_info(const clw_batch* b, uint32_t r, clw_range_stats* stats);

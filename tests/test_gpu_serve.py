@@ -38,7 +38,7 @@ def test_serving_alone_vs_mix(dev, streams):
     about three cohorts at a time: every result equals the request run alone
     (the session path), and the budget is never exceeded."""
     reqs = [Request(i, 0.025 * i, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
-    budget = 30 << 20
+    budget = streams * Server.SCRATCH_SLOTS * 16 + (30 << 20)  # per-stream scratch + ~3 cohorts
     srv = Server(hbm_budget=budget, streams=streams, step_seconds=0.01, admit_interval=0.05)
     try:
         rep = srv.serve(reqs, duration=100.0, until_done=True)
