@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["clw_kernels.hip", "clw_runtime.hip", "cl_wilson.hip", "clw_batch.hip"]
-EXE_SOURCES = ["main.cpp", "putil.cpp"]
+EXE_SOURCES = ["main.cpp", "putil.cpp", "boinc_stub.cpp"]
 
 
 def hipcc() -> str:

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "boinc_stub.h"
 #include "clwilson.h"
 #include "putil.h"
 
@@ -101,7 +102,10 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
     std::vector<uint64_t> lo(n), hi(n);
     clw_work_status ws;
     int which = -1, res = 0;
-    if (clw_ckpt_read(kStateA, kStateB, a.pmin, a.pmax, (uint32_t)n, &ws, lo.data(), hi.data(), &which, &res) !=
+    char pa[512], pb[512];
+    boinc_resolve_filename(kStateA, pa, sizeof(pa));
+    boinc_resolve_filename(kStateB, pb, sizeof(pb));
+    if (clw_ckpt_read(pa, pb, a.pmin, a.pmax, (uint32_t)n, &ws, lo.data(), hi.data(), &which, &res) !=
         CLW_OK) {
       printf("error: %s\n", clw_last_error());
       clw_session_destroy(S);
@@ -130,25 +134,29 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
     clw_range_stats cur;
     clw_session_info(S, &cur);
     clw_work_status ws = {a.pmin, a.pmax, cp, (uint64_t)trickle, 0, cur.totalcount, (uint32_t)n, done};
-    if (clw_ckpt_write(write_a_next ? kStateA : kStateB, &ws, lo.data(), hi.data()) == CLW_OK)
+    char path[512];
+    boinc_resolve_filename(write_a_next ? kStateA : kStateB, path, sizeof(path));
+    if (clw_ckpt_write(path, &ws, lo.data(), hi.data()) == CLW_OK)
       write_a_next = !write_a_next;
     else
       fprintf(stderr, "Cannot write checkpoint to file. Continuing...\n");
+    boinc_checkpoint_completed();
     return 0;
   };
-  // Segment loop in windows, a checkpoint at least every 60 s
-  // (cl_wilson.cpp:1856-1929).
-  time_t last_ckpt = time(nullptr);
+  // Segment loop in windows; progress to the client and a checkpoint when it
+  // asks for one (cl_wilson.cpp:1856-1929, 290-320).
   const uint64_t window = 8ull * 268492800ull;
   int rc = CLW_OK;
   for (uint64_t qlo = currp; rc == CLW_OK && qlo <= info.maxtarget;) {
     const uint64_t qhi = std::min<uint64_t>(qlo + window, info.maxtarget + 1);
+    boinc_begin_critical_section();
     rc = clw_session_run(S, qlo, qhi);
+    boinc_end_critical_section();
     qlo = qhi;
-    if (rc == CLW_OK && !a.test && time(nullptr) - last_ckpt > 60) {
-      last_ckpt = time(nullptr);
+    boinc_fraction_done((double)qlo / (double)(info.maxtarget + 1));
+    if (rc == CLW_OK && !a.test && boinc_time_to_checkpoint()) {
       checkpoint(qlo, 0);
-      if (!quiet) printf("Checkpoint, Current P: %" PRIu64 "\n", qlo);
+      if (!quiet) printf("Checkpoint, Current P: %" PRIu64 ", %.1f%% done\n", qlo, 100.0 * boinc_get_fraction_done());
     }
   }
   if (rc == CLW_OK) rc = clw_session_iterate(S);
@@ -162,7 +170,9 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   }
   // results.txt: one line per (near-)Wilson prime, then the checksum
   // (cl_wilson.cpp:657-688, 1093-1143).
-  FILE* out = fopen(kResultFile, "w");
+  char rpath[512];
+  boinc_resolve_filename(kResultFile, rpath, sizeof(rpath));
+  FILE* out = boinc_fopen(rpath, "w");
   if (!out) {
     printf("Cannot open %s !!!\n", kResultFile);
     return 1;
@@ -310,10 +320,11 @@ int main(int argc, char** argv) {
       return 2;
     }
   }
+  boinc_init();  // main.cpp:169-173
   clw_ctx* ctx = nullptr;
   if (clw_init(a.device, &ctx) != CLW_OK) {
     fprintf(stderr, "error: %s\n", clw_last_error());
-    return 1;
+    return boinc_finish(1);
   }
   clw_device_info info;
   clw_get_device_info(ctx, &info);
@@ -327,5 +338,5 @@ int main(int argc, char** argv) {
     rc = run_range(ctx, a, &st, false);
   }
   clw_shutdown(ctx);
-  return rc;
+  return boinc_finish(rc);  // main.cpp:328
 }
