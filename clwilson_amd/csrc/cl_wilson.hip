@@ -199,7 +199,7 @@ struct clw_session {
 
   ~clw_session() {
     if (ev_ok)
-      for (hipEvent_t e : ev) hipEventDestroy(e);
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);  // teardown: nothing to report
     clw_tp_table_free(ctx, &tab);
     clw_seg_free(ctx, &seg);
     clw_free(ctx, part_lo);
@@ -236,7 +236,7 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
     return clw_fail(CLW_ERR_ARG, "range width %llu > 10000000", (unsigned long long)(pmax - pmin));
   clw_session* S = new clw_session();
   S->ctx = ctx;
-  S->opt = opt ? *opt : clw_options{0, 0, 0, 0, 0};
+  S->opt = opt ? *opt : clw_options{};
   S->pmin = pmin;
   S->pmax = pmax;
   {
@@ -372,9 +372,9 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
     uint32_t nblk_t[3] = {0, 0, 0};
     if (have_range) {
       uint32_t nb = 0;
-      hipEventRecord(S->ev[0], st);
+      CK(clw_hip(hipEventRecord(S->ev[0], st), "hipEventRecord"));
       CK(clw_sieve_segment(ctx, &S->seg, lo, hi, S->T, &nb));
-      hipEventRecord(S->ev[1], st);
+      CK(clw_hip(hipEventRecord(S->ev[1], st), "hipEventRecord"));
       ran_sieve = true;
     }
     for (uint32_t t = 0; t < 3; ++t) {
@@ -387,7 +387,7 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       ran_t[t] = true;
       ran_rec[t] = nblk > 0;
       nblk_t[t] = nblk;
-      hipEventRecord(e4[0], st);
+      CK(clw_hip(hipEventRecord(e4[0], st), "hipEventRecord"));
       uint32_t nparts = 0, Ks = 1;
       if (S->opt.mode == 3) {
         // one test prime per launch pair, as the reference's segment loop
@@ -419,13 +419,13 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
           nparts = K;
         }
       }
-      hipEventRecord(e4[1], st);
+      CK(clw_hip(hipEventRecord(e4[1], st), "hipEventRecord"));
       CK(clw_mul_side(ctx, &S->tab, &S->seg, t, S->off[t], n, Ks, S->part_lo + (size_t)nparts * n,
                       S->part_hi + (size_t)nparts * n, n));
-      hipEventRecord(e4[2], st);
+      CK(clw_hip(hipEventRecord(e4[2], st), "hipEventRecord"));
       nparts += Ks;
       CK(clw_reduce(ctx, &S->tab, S->off[t], n, nparts, S->part_lo, S->part_hi, n));  // reduce.cl
-      hipEventRecord(e4[3], st);
+      CK(clw_hip(hipEventRecord(e4[3], st), "hipEventRecord"));
     }
     // clearn.cl: add the segment's prime count; check the device flags.
     uint32_t flags[4] = {0, 0, 0, 0};

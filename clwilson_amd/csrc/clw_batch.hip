@@ -152,10 +152,11 @@ struct clw_batch {
   } pending;            // window launched by clw_batch_launch, not yet waited for
 
   ~clw_batch() {
-    for (hipEvent_t e : events) hipEventDestroy(e);
-    if (done_ev) hipEventDestroy(done_ev);
-    if (graph_exec) hipGraphExecDestroy(graph_exec);
-    if (graph) hipGraphDestroy(graph);
+    // teardown: errors here have nobody to report to
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (done_ev) (void)hipEventDestroy(done_ev);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (Band& b : bands) {
       if (!b.L) continue;
       clw_tp_table_free(ctx, &b.tab);
@@ -385,8 +386,8 @@ extern "C" int clw_batch_set_scratch(clw_batch* B, uint64_t* part_lo, uint64_t* 
   }
   if (off > cap) return clw_fail(CLW_ERR_ARG, "scratch too small for the bands of this batch");
   // kernel arguments changed: capture the segment step again
-  if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
-  if (B->graph) hipGraphDestroy(B->graph);
+  if (B->graph_exec) CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
+  if (B->graph) CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
   B->graph_exec = nullptr;
   B->graph = nullptr;
   return CLW_OK;
@@ -600,9 +601,9 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   CK(clw_memset(ctx, B->totals, 0, 32));
   for (uint32_t k = 0; k + 4 <= B->ev_used; k += 4) {
     float a = 0, b = 0, c = 0;
-    hipEventElapsedTime(&a, B->events[k], B->events[k + 1]);
-    hipEventElapsedTime(&b, B->events[k + 1], B->events[k + 2]);
-    hipEventElapsedTime(&c, B->events[k + 2], B->events[k + 3]);
+    CK(clw_hip(hipEventElapsedTime(&a, B->events[k], B->events[k + 1]), "hipEventElapsedTime"));
+    CK(clw_hip(hipEventElapsedTime(&b, B->events[k + 1], B->events[k + 2]), "hipEventElapsedTime"));
+    CK(clw_hip(hipEventElapsedTime(&c, B->events[k + 2], B->events[k + 3]), "hipEventElapsedTime"));
     B->ks.sieve_ms += a;
     B->ks.mul_ms += b;
     B->ks.reduce_ms += c;
@@ -837,19 +838,20 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   CK(clw_memset(ctx, B->totals, 0, 32));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, clw_stream(ctx));
-  int rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
-                          (uint32_t)(bd.L - 2), 0, bd.resident, bd.part_lo, bd.part_hi, B->totals + 2);
-  hipEventRecord(e1, clw_stream(ctx));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = clw_hip(hipEventCreate(&e0), "hipEventCreate");
+  if (rc == CLW_OK) rc = clw_hip(hipEventCreate(&e1), "hipEventCreate");
+  if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e0, clw_stream(ctx)), "hipEventRecord");
+  if (rc == CLW_OK)
+    rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
+                        (uint32_t)(bd.L - 2), 0, bd.resident, bd.part_lo, bd.part_hi, B->totals + 2);
+  if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e1, clw_stream(ctx)), "hipEventRecord");
   uint64_t tot[4] = {0, 0, 0, 0};
   if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, 32, 1);
   float ms = 0;
-  hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (rc == CLW_OK) rc = clw_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   CK(rc);
   CK(clw_memset(ctx, B->totals, 0, 32));
   *rate = ms > 0 ? (double)tot[3] / (ms * 1e-3) : 0.0;
@@ -933,8 +935,8 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
     nrep++;
   }
   // the product kernels may have changed: capture the segment step again
-  if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
-  if (B->graph) hipGraphDestroy(B->graph);
+  if (B->graph_exec) CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
+  if (B->graph) CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
   B->graph_exec = nullptr;
   B->graph = nullptr;
   return CLW_OK;

@@ -22,6 +22,8 @@ int clw_fail(int code, const char* fmt, ...);
 int clw_fail_hip(hipError_t e, const char* what);
 // Checks hipGetLastError() after a launch.
 int clw_check_launch(const char* kernel);
+// CLW_OK, or the HIP error recorded for clw_last_error().
+static inline int clw_hip(hipError_t e, const char* what) { return e == hipSuccess ? CLW_OK : clw_fail_hip(e, what); }
 static inline hipStream_t clw_stream(clw_ctx* ctx) { return ctx->stream; }
 int clw_upload_sieve_tables();
 

@@ -10,6 +10,12 @@
 
 #include "clw_internal.h"
 
+#define CK_HIP_OR_RETURN(expr, what)                              \
+  do {                                                            \
+    const hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return clw_fail_hip(e_, what);          \
+  } while (0)
+
 static thread_local char g_err[512] = "";
 
 int clw_fail(int code, const char* fmt, ...) {
@@ -65,7 +71,7 @@ extern "C" int clw_init(int device_index, clw_ctx** out) {
     if (!g_tables_done[device_index]) {
       const int rc = clw_upload_sieve_tables();
       if (rc != CLW_OK) {
-        hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->stream);  // already failing: report the first error
         delete c;
         return rc;
       }
@@ -78,11 +84,12 @@ extern "C" int clw_init(int device_index, clw_ctx** out) {
 
 extern "C" int clw_shutdown(clw_ctx* ctx) {
   if (!ctx) return CLW_OK;
-  hipSetDevice(ctx->device);
-  hipError_t e = hipStreamSynchronize(ctx->stream);
-  hipStreamDestroy(ctx->stream);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  const hipError_t d = hipStreamDestroy(ctx->stream);
   delete ctx;
   if (e != hipSuccess) return clw_fail_hip(e, "shutdown sync");
+  if (d != hipSuccess) return clw_fail_hip(d, "hipStreamDestroy");
   return CLW_OK;
 }
 
@@ -102,7 +109,7 @@ extern "C" int clw_alloc(clw_ctx* ctx, size_t bytes, void** dptr) {
   if (!ctx || !dptr) return clw_fail(CLW_ERR_ARG, "clw_alloc: null argument");
   *dptr = nullptr;
   if (bytes == 0) bytes = 16;
-  hipSetDevice(ctx->device);
+  CK_HIP_OR_RETURN(hipSetDevice(ctx->device), "hipSetDevice");
   const hipError_t e = hipMalloc(dptr, bytes);
   if (e != hipSuccess) {
     *dptr = nullptr;
@@ -114,7 +121,7 @@ extern "C" int clw_alloc(clw_ctx* ctx, size_t bytes, void** dptr) {
 
 extern "C" int clw_free(clw_ctx* ctx, void* dptr) {
   if (!dptr) return CLW_OK;
-  if (ctx) hipSetDevice(ctx->device);
+  if (ctx) CK_HIP_OR_RETURN(hipSetDevice(ctx->device), "hipSetDevice");
   const hipError_t e = hipFree(dptr);
   if (e != hipSuccess) return clw_fail_hip(e, "hipFree");
   return CLW_OK;
@@ -196,9 +203,9 @@ extern "C" int clw_event_elapsed_ms(clw_event* a, clw_event* b, float* ms) {
 
 extern "C" int clw_event_destroy(clw_event* ev) {
   if (!ev) return CLW_OK;
-  hipEventDestroy(ev->ev);
+  const hipError_t e = hipEventDestroy(ev->ev);
   delete ev;
-  return CLW_OK;
+  return clw_hip(e, "hipEventDestroy");
 }
 
 extern "C" int clw_limbs_for(uint64_t p_max) {

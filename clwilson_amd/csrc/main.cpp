@@ -86,7 +86,8 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   p.resize(n);
   q.resize(n);
   r.resize(n);
-  clw_options opt = {0, 0, quiet ? 0u : 1u, 0, 0};
+  clw_options opt{};
+  opt.verbose = quiet ? 0u : 1u;
   clw_session* S = nullptr;
   if (clw_session_create(ctx, a.pmin, a.pmax, &opt, &S) != CLW_OK) {
     printf("error: %s\n", clw_last_error());
