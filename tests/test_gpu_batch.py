@@ -229,3 +229,28 @@ def test_batch_checkpoint_resume(dev, tmp_path):
             _same(b.results(r), ref[r])
     with cw.Batch(dev, ranges[:3]) as b:  # other ranges: no valid file
         assert b.resume(a, bp)[0] == 0
+
+
+def test_batch_edge_ranges(dev):
+    """Ranges at the edges: only p = 5 (target 1, nothing to sieve), a single
+    prime, ranges straddling the 2/3-limb band boundary (7.59e8), a range of
+    one type only, all in one batch, each equal to its own run."""
+    ranges = [(5, 6), (5, 12), (97, 98), (759250000, 759260000), (759250120, 759250200), (1009, 1014)]
+    with cw.Batch(dev, ranges) as b:
+        assert [st.limbs for st in b.info][3:5] == [3, 3]
+        got = b.search()
+    for (lo, hi), r in zip(ranges, got):
+        _same(r, dev.wilson_range(lo, hi))
+
+
+def test_batch_band_boundary_3_4_limbs(dev):
+    """Test primes on both sides of the 3/4-limb boundary (2.4879e13): window
+    residues equal the session path's."""
+    ranges = [(24879108090000, 24879108095803), (24879108095803, 24879108100000)]
+    with cw.Batch(dev, ranges) as b:
+        assert [st.limbs for st in b.info] == [3, 4]
+        b.run(2, 300_000_000)
+        for r, (lo, hi) in enumerate(ranges):
+            with cw.Session(dev, lo, hi) as s:
+                s.run(2, 300_000_000)
+                assert b.residues(r) == s.residues()
