@@ -91,6 +91,8 @@ def lib():
         L.orc_finalize_ex.argtypes = [u64, u32, u64, u64, i64, i64, p64]
         L.orc_factorial_mod_p2_direct.restype = None
         L.orc_factorial_mod_p2_direct.argtypes = [u64, u64, p64]
+        L.orc_direct_quotients.restype = None
+        L.orc_direct_quotients.argtypes = [p64, u64, p64, p64, C.c_int]
         L.orc_window_product.restype = None
         L.orc_window_product.argtypes = [p64, p64, u32, u64, u64, C.c_int, p64, p64]
         L.orc_iterate.restype = None
@@ -172,6 +174,17 @@ def wilson_range(pmin: int, pmax: int, nthreads: int | None = None):
                  lastw=st.lastw, tpcount=st.tpcount, tpcnt=list(st.tpcnt),
                  resultcount=st.resultcount)
     return dict(p=p[:n], type=ty[:n], quot=quot[:n], rem=rem[:n]), stats
+
+
+def direct_quotients(ps: np.ndarray, nthreads: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """(quot, rem) of (p-1)!+1 = rem + p quot mod p^2 from the definition
+    (the product 2 * 3 * ... * (p-1)), for p < 2^31."""
+    ps = np.ascontiguousarray(ps, dtype=np.uint64)
+    assert len(ps) == 0 or int(ps.max()) < 2**31
+    q = np.zeros(max(len(ps), 1), dtype=np.uint64)
+    r = np.zeros(max(len(ps), 1), dtype=np.uint64)
+    lib().orc_direct_quotients(_p64(ps), len(ps), _p64(q), _p64(r), nthreads or nthreads_default())
+    return q[: len(ps)], r[: len(ps)]
 
 
 def wilson_quotient(p: int, quot: int) -> int:

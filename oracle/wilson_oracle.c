@@ -458,6 +458,44 @@ void orc_factorial_mod_p2_direct(uint64_t p, uint64_t n, uint64_t out[2]) {
   out[0] = r; out[1] = 0;
 }
 
+/* Direct Wilson quotients for many primes: (p-1)! mod p^2 by multiplying
+ * 2, 3, ..., p-1 one after another (the textbook definition, nothing of the
+ * plain method), then quot = ((p-1)!+1)/p mod p and rem = ((p-1)!+1) mod p.
+ * Montgomery products mod m = p^2 with R = 2^64 keep it to a few ns per
+ * factor: every product of a plain k adds R^-1, removed at the end with
+ * R^(p-2) mod m. p < 2^31 (m < 2^62), OpenMP over the primes. */
+static uint64_t mont64_inv(uint64_t m) { /* -m^-1 mod 2^64, m odd */
+  uint64_t x = m;
+  for (int i = 0; i < 6; ++i) x *= 2 - m * x;
+  return (uint64_t)0 - x;
+}
+static inline uint64_t mont64_mul(uint64_t a, uint64_t b, uint64_t m, uint64_t minv) {
+  const u128 t = (u128)a * b;
+  const uint64_t q = (uint64_t)t * minv;
+  const u128 u = (t + (u128)q * m) >> 64;
+  return (uint64_t)(u >= m ? u - m : u);
+}
+void orc_direct_quotients(const uint64_t* ps, uint64_t n, uint64_t* quot, uint64_t* rem, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t p = ps[i], m = p * p, minv = mont64_inv(m);
+    uint64_t r = 1;
+    for (uint64_t k = 2; k < p; ++k) r = mont64_mul(r, k, m, minv); /* (p-1)! R^-(p-2) */
+    /* R^(p-2) mod m by square and multiply with u128 remainders */
+    const uint64_t Rm = (uint64_t)(((u128)1 << 64) % m);
+    uint64_t fix = 1 % m, b = Rm, e = p - 2;
+    while (e) {
+      if (e & 1) fix = (uint64_t)(((u128)fix * b) % m);
+      b = (uint64_t)(((u128)b * b) % m);
+      e >>= 1;
+    }
+    const uint64_t f = (uint64_t)(((u128)r * fix) % m); /* (p-1)! mod p^2 */
+    const uint64_t w = (f + 1) % m;
+    quot[i] = w / p;
+    rem[i] = w % p;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* The plain method for one range (cl_wilson.cpp:1442-1974).                 */
 /* ------------------------------------------------------------------------ */

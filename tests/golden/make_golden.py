@@ -1,6 +1,6 @@
 """Writes the golden fixtures of tests/golden/ with the CPU oracle.
 
-    python tests/golden/make_golden.py [--cfg1]
+    python tests/golden/make_golden.py [--cfg1] [--direct]
 
 Fixtures (JSON):
   kat_selftest.json   the reference's seven self-test answers (cl_wilson.cpp:2007-2123)
@@ -9,6 +9,10 @@ Fixtures (JSON):
   windows.json        canonical window products at config-2/3 magnitudes
   cfg1.json           (--cfg1) the whole of [1e6, 2e6): checksum, results, sha256
                       of the Wilson quotient vector (little-endian uint64)
+  direct_lt_2e6.json  (--direct) every prime p < 2e6 by the definition,
+                      (p-1)! = 2 * 3 * ... * (p-1) mod p^2: per range [5, 1e6)
+                      and [1e6, 2e6) the count, checksum, sha256 of the
+                      quotient vector, and every 500th (p, quot) as samples
 """
 from __future__ import annotations
 
@@ -115,6 +119,8 @@ def acu_values(p, t):
 
 
 def main():
+    if sys.argv[1:] == ["--direct"]:
+        return direct_only()
     dump("kat_selftest.json", [dict(pmin=a, pmax=b, results=c, checksum=f"{d:016X}", pi_maxt=e, lastp=f,
                                     lastw=g) for a, b, c, d, e, f, g in KATS])
     dump("cgh_tables.json", dict(table1=CGH_T1, table2=CGH_T2))
@@ -170,6 +176,22 @@ def main():
                                resultcount=st["resultcount"], results=res,
                                quot_sha256=quot_hash(arr["quot"]), p_sha256=quot_hash(arr["p"])))
         print("cfg1", round(time.time() - t, 1), hex(st["checksum"]), flush=True)
+    if "--direct" in sys.argv:
+        direct_only()
+
+
+def direct_only():
+    out = []
+    for lo, hi in [(5, 1000000), (1000000, 2000000)]:
+        t = time.time()
+        ps = O.primes_range(lo, hi)
+        q, r = O.direct_quotients(ps)
+        assert not r.any()
+        checksum = int((ps.astype(object) + q.astype(object) + r.astype(object)).sum()) % 2**64
+        out.append(dict(pmin=lo, pmax=hi, tpcount=len(ps), checksum=f"{checksum:016X}", quot_sha256=quot_hash(q),
+                        samples=[[int(ps[i]), int(q[i])] for i in range(0, len(ps), 500)]))
+        print("direct", lo, hi, round(time.time() - t, 1), flush=True)
+    dump("direct_lt_2e6.json", out)
 
 
 if __name__ == "__main__":

@@ -186,3 +186,18 @@ def test_config2_full_range(dev, golden):
     assert applied == g["pi_maxt"]  # every prime up to the largest target, once
     assert len(r.p) == g["tpcount"] and r.checksum_hex == g["checksum"] and r.results == g["results"]
     assert hashlib.sha256(np.ascontiguousarray(r.quot, dtype="<u8").tobytes()).hexdigest() == g["quot_sha256"]
+
+
+@pytest.mark.parametrize("engine", ["session", "batch"])
+def test_every_prime_below_2e6_vs_definition(dev, golden, engine):
+    """Every prime p < 2e6 against (p-1)! = 2*3*...*(p-1) mod p^2 computed
+    by definition (direct_lt_2e6.json): quotient-vector hash and checksum of
+    [5, 1e6) and [1e6, 2e6), through the session path and the batched engine."""
+    for d in golden("direct_lt_2e6.json"):
+        if engine == "session":
+            r = dev.wilson_range(d["pmin"], d["pmax"])
+        else:
+            with cw.Batch(dev, [(d["pmin"], d["pmax"])]) as b:
+                r = b.search()[0]
+        assert len(r.p) == d["tpcount"] and r.checksum_hex == d["checksum"]
+        assert hashlib.sha256(np.ascontiguousarray(r.quot, dtype="<u8").tobytes()).hexdigest() == d["quot_sha256"]

@@ -226,3 +226,23 @@ def test_golden_small_consistent(golden):
         arr, st = O.wilson_range(case["pmin"], case["pmax"])
         assert [int(x) for x in arr["quot"]] == case["quot"]
         assert f"{st['checksum']:016X}" == case["checksum"]
+
+
+def test_direct_fixture_matches_definition_and_plain_method(golden):
+    """direct_lt_2e6.json holds every prime p < 2e6 by the definition
+    (p-1)! = 2*3*...*(p-1) mod p^2: its samples are recomputed here with
+    Python integers (p < 2e4) and the oracle's direct product, and on
+    [1e6, 2e6) its quotient hash and checksum equal cfg1.json, which the
+    plain-method oracle wrote -- two independent algorithms agree on all
+    70,435 primes."""
+    d = golden("direct_lt_2e6.json")
+    cfg1 = golden("cfg1.json")
+    assert [x["tpcount"] for x in d] == [78496, 70435]  # pi(1e6) - 2, pi(2e6) - pi(1e6)
+    big = d[1]
+    assert (big["quot_sha256"], big["checksum"]) == (cfg1["quot_sha256"], cfg1["checksum"])
+    samples = np.array([p for p, _ in d[0]["samples"] + d[1]["samples"]], dtype=np.uint64)
+    q, r = O.direct_quotients(samples)
+    assert [int(x) for x in q] == [w for _, w in d[0]["samples"] + d[1]["samples"]] and not r.any()
+    for p, w in d[0]["samples"][:40]:
+        if p < 20000:
+            assert (_fact_mod(p - 1, p * p) + 1) % (p * p) == w * p
