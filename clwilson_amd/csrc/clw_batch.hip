@@ -893,7 +893,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
       const double per_block = (double)active * CLW_BLOCK_SPAN / std::log((double)q);
       const uint64_t blocks = std::min<uint64_t>(B->max_blocks - 2, std::max<uint64_t>(4, (uint64_t)(2e11 / per_block) + 1));
       // candidates: test primes per lane with a k_batch_mul instantiation
-      static const uint32_t kps_by_band[5][6] = {{0}, {0}, {1, 2, 3, 4}, {2, 3, 4, 5, 6}, {1, 2, 3, 4}};
+      static const uint32_t kps_by_band[5][6] = {{0}, {0}, {1, 2, 3, 4}, {2, 3, 4, 5, 6}, {1, 2, 3, 4, 5}};
       std::vector<uint32_t> kps;
       for (uint32_t a : kps_by_band[bd.L])
         if (a) kps.push_back(a);

@@ -1087,6 +1087,7 @@ extern "C" int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t
     case 4 * 8 + 2: { constexpr int LL = 4, KK = 2; CALL; } break;                                        \
     case 4 * 8 + 3: { constexpr int LL = 4, KK = 3; CALL; } break;                                        \
     case 4 * 8 + 4: { constexpr int LL = 4, KK = 4; CALL; } break;                                        \
+    case 4 * 8 + 5: { constexpr int LL = 4, KK = 5; CALL; } break;                                        \
     default: return clw_fail(CLW_ERR_ARG, "batch: unsupported limbs %u / primes per lane %u", (unsigned)(L), \
                              (unsigned)(KP));                                                             \
   }

@@ -349,7 +349,7 @@ int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, ui
 /* Autotuner of the product kernel per limb band: times, on one trial
  * segment of sieving primes at magnitude q (sieve + product kernel into
  * scratch; no residue changes), every candidate of test primes per lane
- * (1-4 for 2 and 4 limbs, 2-6 for 3 limbs: the number of Montgomery products
+ * (1-4 for 2 limbs, 2-6 for 3, 1-5 for 4 limbs: the number of Montgomery products
  * a lane runs in lockstep, which sets the instruction schedule and the
  * work-group tile of 256 x per_lane test primes) x work items per resident
  * block (2, 8, 32:

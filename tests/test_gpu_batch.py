@@ -166,15 +166,16 @@ def test_autotune_keeps_results(dev, tmp_path):
 @pytest.mark.parametrize("per_lane", [1, 2, 3, 4, 5, 6])
 def test_batch_every_per_lane_variant(dev, per_lane):
     """Every instantiation the autotuner may pick (test primes per lane)
-    gives the same results: 2 and 3 limbs by full searches, 4 limbs by window
-    residues against the session path."""
+    gives the same results: 2 and 3 limbs by full searches (5-6 per lane: 3
+    limbs only), 4 limbs (1-5 per lane) by window residues against the
+    session path."""
     ranges = MIX[5:8] if per_lane > 4 else MIX[3:8]  # 5-6 per lane: 3 limbs only
     say("per_lane", per_lane)
     with cw.Batch(dev, ranges, per_lane=per_lane) as b:
         got = b.search()
     for (lo, hi), r in zip(ranges, got):
         _same(r, dev.wilson_range(lo, hi))
-    if per_lane <= 4:
+    if per_lane <= 5:  # 4-limb variants: 1-5 per lane
         lo4, hi4 = 10**14, 10**14 + 3000
         with cw.Batch(dev, [(lo4, hi4)], per_lane=per_lane) as b, cw.Session(dev, lo4, hi4) as s:
             b.run(40000000, 900000000)
