@@ -187,6 +187,7 @@ def measure_plain(dev, cw, regs, nxt, steps):
     """The reference's launch structure on the same workload: one test prime
     per launch pair (mullarge + reduce), one window per region."""
     with cw.Session(dev, PMIN, PMAX, mode=3) as s:
+        ntp = s.n
         rates = []
         for ri in range(min(steps, 3)):
             hi = nxt[ri]
@@ -204,7 +205,7 @@ def measure_plain(dev, cw, regs, nxt, steps):
         if len(rates) < 3:
             mean = sum(rates) / len(rates)
             total += sum(F / mean for (_, _, _, _, F) in regs[len(rates):])
-    return dict(value=327107 / total, region_rates_factor_apps_per_s=rates,
+    return dict(value=ntp / total, region_rates_factor_apps_per_s=rates,
                 structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
 
 
@@ -354,6 +355,9 @@ def main() -> int:
                          "autotuner finds within noise of the best at 3 limbs)")
     ap.add_argument("--serve-seconds", type=float, default=45.0,
                     help="wall-clock window of the serving replay (0 = skip it)")
+    ap.add_argument("--range", type=int, nargs=2, metavar=("PMIN", "PMAX"), default=None,
+                    help="measure another range the same way (e.g. 1e11 1e11+1e7 to check the extrapolation "
+                         "against a full search); the JSON line then describes that range")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: a 256 MiB device copy (counter calibration), then two "
                          "one-segment windows of the [2e13, 2e13+1e7) batch (two k_batch_mul launches)")
@@ -365,6 +369,9 @@ def main() -> int:
 
     import clwilson_amd as cw
 
+    global PMIN, PMAX
+    if args.range:
+        PMIN, PMAX = args.range
     if args.profile_run:
         return profile_run(cw, args)
     dev = cw.Device(0)
@@ -443,9 +450,11 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "exact: every prime in [2e13, 2e13+1e7); windows of the sieving-prime axis, extrapolated",
+        "data": f"exact: every prime in [{PMIN}, {PMAX}); windows of the sieving-prime axis, extrapolated",
         "config": {
-            "workload": "every prime p in [2e13, 2e13+1e7) (production magnitude, p^2 ~ 2^89)",
+            "workload": ("every prime p in [2e13, 2e13+1e7) (production magnitude, p^2 ~ 2^89)"
+                         if (PMIN, PMAX) == (20_000_000_000_000, 20_000_010_000_000)
+                         else f"every prime p in [{PMIN}, {PMAX})"),
             "engine": run.name,
             "test_primes": sum(cnt), "types": cnt, "type_targets": T, "limbs": L,
             "step": f"one segment of {SPAN} sieving integers for all test primes",
@@ -465,7 +474,7 @@ def main() -> int:
             "frac": achieved / peak if peak else None,
             "traffic": (BATCH_MUL_TRAFFIC["fetch_kib"] * BATCH_MUL_TRAFFIC["fetch_scale"]
                         + BATCH_MUL_TRAFFIC["write_kib"] * BATCH_MUL_TRAFFIC["write_scale"]) * 1024
-            if run.kernel == "k_batch_mul<3,4>" else None,
+            if run.kernel == "k_batch_mul<3,4>" and args.range is None else None,
             "traffic_of": BATCH_MUL_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
             "per_unit": f"{prod_per_rec} 30x30-bit partial products per record per test prime",
             "launches": ks["mul_launches"],
