@@ -115,24 +115,20 @@ CLW_HD void pin_all(uint64_t (&c)[K]) {
 template <int L>
 constexpr int unmasked_m() { return (L == 3 || L == 4) ? 2 : 1; }
 
-// G x K independent Montgomery products in lockstep: chain j = g K + t
-// computes r[j] = a[j] * b[g] * R^-1 mod n[t], with wave-uniform multipliers
-// b[g]. The chains run column by column side by side, so consecutive
-// instructions belong to different chains and the 64-bit accumulators never
-// wait on each other and need no extra adds (G = 2 gives each test prime two
-// accumulators fed by alternate factors: twice the independent chains for
-// the same N and -N^-1 registers).
-// Requires a[j] < 2 n[t] and b[g] < n[t] / 4 (every record factor Q <= T^2 <
-// p^2 / 4); then r[j] < 2 n[t] although m may exceed R by ~3 2^(30 U):
+// K independent Montgomery products r[j] = a[j] * b * R^-1 mod n[j] with a
+// common (wave-uniform) multiplier b, computed column by column in lockstep:
+// consecutive instructions belong to different test primes, so the 64-bit
+// accumulator chains never stall on each other and need no extra adds.
+// Requires a[j] < 2 n[j] and b < n[j] / 4 (every record factor Q <= T^2 <
+// p^2 / 4); then r[j] < 2 n[j] although m may exceed R by ~3 2^(30 U):
 // (a b + m n) / R < n/4 + n (1 + 2^-28) < 2 n.
-template <int L, int K, int G>
-CLW_HD void montmul_lockstep_g(Res<L> (&r)[G * K], const Res<L> (&a)[G * K], const Res<L> (&b)[G],
-                               const Res<L> (&n)[K], const uint32_t (&ninv)[K]) {
-  constexpr int C = G * K;
-  uint32_t m[C][L];
-  uint64_t c[C];
+template <int L, int K>
+CLW_HD void montmul_lockstep(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, const Res<L> (&n)[K],
+                             const uint32_t (&ninv)[K]) {
+  uint32_t m[K][L];
+  uint64_t c[K];
 #pragma unroll
-  for (int j = 0; j < C; ++j) c[j] = 0;
+  for (int j = 0; j < K; ++j) c[j] = 0;
 #pragma unroll
   for (int k = 0; k < 2 * L - 1; ++k) {
 #pragma unroll
@@ -140,8 +136,8 @@ CLW_HD void montmul_lockstep_g(Res<L> (&r)[G * K], const Res<L> (&a)[G * K], con
       const int jj = k - i;
       if (jj >= 0 && jj < L) {
 #pragma unroll
-        for (int j = 0; j < C; ++j) c[j] = mad32(a[j].v[i], b[j / K].v[jj], c[j]);
-        pin_all<C>(c);
+        for (int j = 0; j < K; ++j) c[j] = mad32(a[j].v[i], b.v[jj], c[j]);
+        pin_all<K>(c);
       }
     }
 #pragma unroll
@@ -149,37 +145,29 @@ CLW_HD void montmul_lockstep_g(Res<L> (&r)[G * K], const Res<L> (&a)[G * K], con
       const int jj = k - i;
       if (i < k && jj >= 0 && jj < L) {
 #pragma unroll
-        for (int j = 0; j < C; ++j) c[j] = mad32(m[j][i], n[j % K].v[jj], c[j]);
-        pin_all<C>(c);
+        for (int j = 0; j < K; ++j) c[j] = mad32(m[j][i], n[j].v[jj], c[j]);
+        pin_all<K>(c);
       }
     }
     if (k < L) {
 #pragma unroll
-      for (int j = 0; j < C; ++j) {
-        const uint32_t mk = (uint32_t)c[j] * ninv[j % K];
+      for (int j = 0; j < K; ++j) {
+        const uint32_t mk = (uint32_t)c[j] * ninv[j];
         m[j][k] = k < unmasked_m<L>() ? mk : (mk & LMASK);
-        c[j] = mad32(m[j][k], n[j % K].v[0], c[j]) >> LB;
+        c[j] = mad32(m[j][k], n[j].v[0], c[j]) >> LB;
       }
-      pin_all<C>(c);
+      pin_all<K>(c);
     } else {
 #pragma unroll
-      for (int j = 0; j < C; ++j) {
+      for (int j = 0; j < K; ++j) {
         r[j].v[k - L] = (uint32_t)c[j] & LMASK;
         c[j] >>= LB;
       }
-      pin_all<C>(c);
+      pin_all<K>(c);
     }
   }
 #pragma unroll
-  for (int j = 0; j < C; ++j) r[j].v[L - 1] = (uint32_t)c[j];
-}
-
-// K products with one common multiplier b (G = 1).
-template <int L, int K>
-CLW_HD void montmul_lockstep(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, const Res<L> (&n)[K],
-                             const uint32_t (&ninv)[K]) {
-  const Res<L> bb[1] = {b};
-  montmul_lockstep_g<L, K, 1>(r, a, bb, n, ninv);
+  for (int j = 0; j < K; ++j) r[j].v[L - 1] = (uint32_t)c[j];
 }
 
 // r - n if r >= n, else r (limb-wise borrow).
