@@ -54,7 +54,7 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB
 # for the 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
 # profiles/batch_mul_cfg3_hbm_counters.md.
-BATCH_MUL_TRAFFIC = dict(fetch_kib=1234392.2, write_kib=133782.9, fetch_scale=2.0, write_scale=1.0,
+BATCH_MUL_TRAFFIC = dict(fetch_kib=1424239.8, write_kib=529838.6, fetch_scale=2.0, write_scale=1.0,
                          window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
 
 
@@ -474,7 +474,7 @@ def main() -> int:
             "frac": achieved / peak if peak else None,
             "traffic": (BATCH_MUL_TRAFFIC["fetch_kib"] * BATCH_MUL_TRAFFIC["fetch_scale"]
                         + BATCH_MUL_TRAFFIC["write_kib"] * BATCH_MUL_TRAFFIC["write_scale"]) * 1024
-            if run.kernel == "k_batch_mul<3,4>" and args.range is None else None,
+            if run.kernel == "k_batch_mul<3,4>" and args.range is None and not args.tune_single else None,
             "traffic_of": BATCH_MUL_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
             "per_unit": f"{prod_per_rec} 30x30-bit partial products per record per test prime",
             "launches": ks["mul_launches"],

@@ -45,8 +45,8 @@ const uint64_t kMaxWidth = 10000000;
 const uint32_t kMaxSpanBlocks = 4370;  // largest default segment (268 M integers)
 const uint32_t kMinSpanBlocks = 16;
 const double kAppsPerSegment = 2e12;   // factor applications aimed at per segment
-const uint32_t kItemsPerBlock = 8;     // work items per resident block (balance)
-const uint32_t kMaxChunks = 64;
+const uint32_t kItemsPerBlock = 32;    // work items per resident block (balance)
+const uint32_t kMaxChunks = 256;
 const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 
 inline uint32_t prime_type(uint64_t p) { return p % 3 == 1 ? 0 : (p % 12 == 5 ? 1 : 2); }
@@ -293,7 +293,7 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     size_t tiles = 0;  // at one test prime per lane (the most items any plan has)
     for (const GroupHost& g : bd.groups) tiles += (g.tp_n + 255) / 256;
     const uint64_t chunks = std::min<uint32_t>(B->opt.max_chunks ? B->opt.max_chunks : kMaxChunks, kMaxChunks);
-    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(chunks * n, 1ull << 24));
+    bd.part_cap = std::max<uint64_t>(n, std::min<uint64_t>(chunks * n, 1ull << 26));
     rc = clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident);
     if (rc == CLW_OK && B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
     if (rc == CLW_OK) rc = clw_tp_table_alloc(ctx, n, (uint32_t)bd.L, &bd.tab);
@@ -897,7 +897,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
       std::vector<uint32_t> kps;
       for (uint32_t a : kps_by_band[bd.L])
         if (a) kps.push_back(a);
-      const uint32_t ipbs[3] = {2, 8, 32};
+      const uint32_t ipbs[3] = {8, 32, 128};
       double best = -1, warm = 0;
       bd.per_lane = clw_default_per_lane((uint32_t)bd.L);
       bd.items_per_block = kItemsPerBlock;

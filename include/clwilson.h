@@ -232,7 +232,7 @@ typedef struct {
                               worker threads (no device work while building,
                               e.g. beside busy streams) instead of the device */
   uint32_t max_chunks;     /* batches: chunks a segment may be split into
-                              (chunk-product slots per test prime; 0 = 64)   */
+                              (chunk-product slots per test prime; 0 = 256)  */
   uint32_t external_scratch; /* batches: allocate no chunk-product slots; the
                               caller provides them (clw_batch_set_scratch)   */
 } clw_options;
@@ -352,7 +352,7 @@ int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, ui
  * (1-4 for 2 limbs, 2-6 for 3, 1-5 for 4 limbs: the number of Montgomery products
  * a lane runs in lockstep, which sets the instruction schedule and the
  * work-group tile of 256 x per_lane test primes) x work items per resident
- * block (2, 8, 32:
+ * block (8, 32, 128:
  * the chunk count, i.e. the fan-in of the chunk-product reduction), and keeps
  * the fastest. With cache_path, a tuned (device, band, size class) is read
  * from / appended to that text file instead of re-timed. report receives one
