@@ -48,13 +48,13 @@ METRIC = "primes tested/s at p≈2e13 (batched vs plain), % of int-MAD roofline"
 PMIN, PMAX = 20_000_000_000_000, 20_000_010_000_000
 SPAN = 4370 * 61440  # integers per step (one default segment)
 
-# HBM traffic of k_batch_mul<3,4> on one SPAN segment just below T0 (two
+# HBM traffic of k_batch_mul<3,3> on one SPAN segment just below T0 (two
 # launches of `bench.py --profile-run` under rocprofv3 --pmc FETCH_SIZE, then
 # --pmc WRITE_SIZE; KiB per dispatch). The counters are scaled by what they
 # read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB
 # for the 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
 # profiles/batch_mul_cfg3_hbm_counters.md.
-BATCH_MUL_TRAFFIC = dict(fetch_kib=1424239.8, write_kib=529838.6, fetch_scale=2.0, write_scale=1.0,
+BATCH_MUL_TRAFFIC = dict(fetch_kib=1427308.5, write_kib=494887.6, fetch_scale=2.0, write_scale=1.0,
                          window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
 
 
@@ -115,7 +115,7 @@ class BatchRunner:
         st = self.b.info[0]
         self.T, self.cnt, self.limbs = list(st.typeTarget), list(st.tpcnt), st.limbs
         self.tune = self.b.autotune(st.typeTarget[0]) if tune else None
-        self.kp = self.tune[0]["per_lane"] if tune else (per_lane or int(cw.lib().clw_default_per_lane(self.limbs)))
+        self.kp = self.tune[0]["per_lane"] if tune else (per_lane or int(cw.lib().clw_batch_default_per_lane(self.limbs)))
         self.kernel = f"k_batch_mul<{self.limbs},{self.kp}>"
         self.name = "batched engine (one-range batch, persistent work-queue kernel)"
         self._base = None
@@ -351,8 +351,8 @@ def main() -> int:
                     help="windows timed on the 256-range batch (0 = skip it)")
     ap.add_argument("--no-tune", action="store_true", help="256-range batch without the autotuner")
     ap.add_argument("--tune-single", action="store_true",
-                    help="autotune the one-range batch too (default: 4 test primes per lane, which the "
-                         "autotuner finds within noise of the best at 3 limbs)")
+                    help="autotune the one-range batch too (default: 3 test primes per lane, which the "
+                         "autotuner finds best at 3 limbs)")
     ap.add_argument("--serve-seconds", type=float, default=45.0,
                     help="wall-clock window of the serving replay (0 = skip it)")
     ap.add_argument("--range", type=int, nargs=2, metavar=("PMIN", "PMAX"), default=None,
@@ -474,7 +474,7 @@ def main() -> int:
             "frac": achieved / peak if peak else None,
             "traffic": (BATCH_MUL_TRAFFIC["fetch_kib"] * BATCH_MUL_TRAFFIC["fetch_scale"]
                         + BATCH_MUL_TRAFFIC["write_kib"] * BATCH_MUL_TRAFFIC["write_scale"]) * 1024
-            if run.kernel == "k_batch_mul<3,4>" and args.range is None and not args.tune_single else None,
+            if run.kernel == "k_batch_mul<3,3>" and args.range is None and not args.tune_single else None,
             "traffic_of": BATCH_MUL_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
             "per_unit": f"{prod_per_rec} 30x30-bit partial products per record per test prime",
             "launches": ks["mul_launches"],

@@ -145,6 +145,7 @@ def lib():
         "clw_blocks_upto": (u32, [u64, u64, u64]),
         "clw_mul_records": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, u32, vp, vp, u32, u32]),
         "clw_default_per_lane": (u32, [u32]),
+        "clw_batch_default_per_lane": (u32, [u32]),
         "clw_mul_side": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, vp, vp, u32]),
         "clw_reduce": (i32, [vp, P(TpTable), u32, u32, u32, vp, vp, u32]),
         "clw_iterate": (i32, [vp, P(TpTable), P(u64)]),

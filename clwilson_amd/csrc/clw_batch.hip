@@ -284,7 +284,7 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, 32);
   for (Band& bd : B->bands) {
     if (!bd.L || rc != CLW_OK) continue;
-    bd.per_lane = B->opt.per_lane ? B->opt.per_lane : clw_default_per_lane((uint32_t)bd.L);
+    bd.per_lane = B->opt.per_lane ? B->opt.per_lane : clw_batch_default_per_lane((uint32_t)bd.L);
     const uint32_t n = (uint32_t)bd.p.size();
     if (bd.groups.size() >= 65536) {
       rc = clw_fail(CLW_ERR_ARG, "too many ranges in one limb band of a batch (%zu groups)", bd.groups.size());
@@ -392,6 +392,8 @@ extern "C" int clw_batch_set_scratch(clw_batch* B, uint64_t* part_lo, uint64_t* 
   B->graph = nullptr;
   return CLW_OK;
 }
+
+extern "C" uint32_t clw_batch_default_per_lane(uint32_t limbs) { return limbs <= 2 ? 4 : limbs == 3 ? 3 : 2; }
 
 // Work plan of a window whose first sieving prime is q0: the groups with
 // target >= q0 get items (their tiles), the others none; K chunks per
@@ -899,7 +901,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
         if (a) kps.push_back(a);
       const uint32_t ipbs[3] = {8, 32, 128};
       double best = -1, warm = 0;
-      bd.per_lane = clw_default_per_lane((uint32_t)bd.L);
+      bd.per_lane = clw_batch_default_per_lane((uint32_t)bd.L);
       bd.items_per_block = kItemsPerBlock;
       CK(tune_trial(B, bd, q, blocks, &warm));  // code-object load, clocks up
       for (uint32_t a : kps) {

@@ -170,9 +170,9 @@ uint32_t clw_blocks_upto(uint64_t lo, uint64_t hi, uint64_t T);
 int clw_mul_records(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, uint32_t type,
                     uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K,
                     uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride, uint32_t per_lane);
-/* Test primes per lane of clw_mul_records for a limb schedule (1, 2 or 4;
- * the product kernel interleaves their Montgomery products). 0 in
- * clw_mul_records / clw_options means this default. */
+/* Test primes per lane of clw_mul_records for a limb schedule (4 up to 3
+ * limbs, else 2; the product kernel interleaves their Montgomery products).
+ * 0 in clw_mul_records / clw_options means this default for sessions. */
 uint32_t clw_default_per_lane(uint32_t limbs);
 /* The same product with a wave per test prime whose 64 lanes share the
  * records (for ranges with few test primes); grid (ceil(tp_n/4), K). */
@@ -305,6 +305,11 @@ typedef struct clw_batch clw_batch;
 int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64_t* pmax, uint32_t nranges,
                      const clw_options* opt, clw_batch** out);
 int clw_batch_destroy(clw_batch* b);
+/* Test primes per lane of the batched product kernel for a limb band when
+ * neither clw_options.per_lane nor the autotuner sets it: 4 for 2 limbs, 3
+ * for 3 limbs, 2 for 4 limbs (the fastest at each band's production
+ * magnitude). */
+uint32_t clw_batch_default_per_lane(uint32_t limbs);
 /* Moves the batch to another context (stream) of the same device; no window
  * may be in flight. Lets one thread build batches while others run. */
 int clw_batch_bind(clw_batch* b, clw_ctx* ctx);
