@@ -95,8 +95,12 @@ CLW_HD Res<L> montmul(const Res<L>& a, const Res<L>& b, const Res<L>& n, uint32_
 // different test primes are interleaved instead.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define CLW_PIN1(a) asm volatile("" : "+v"(a))
+#define CLW_PIN32(a) asm volatile("" : "+v"(a))
+#define CLW_PIN_S(a) asm volatile("" : "+s"(a))
 #else
 #define CLW_PIN1(a) ((void)0)
+#define CLW_PIN32(a) ((void)0)
+#define CLW_PIN_S(a) ((void)0)
 #endif
 
 template <int K>
@@ -168,6 +172,117 @@ CLW_HD void montmul_lockstep(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>&
   }
 #pragma unroll
   for (int j = 0; j < K; ++j) r[j].v[L - 1] = (uint32_t)c[j];
+}
+
+// Lazy form of the batched kernel's run accumulators. montmul_lazy is
+// montmul_lockstep with two instructions fewer per 3-limb product:
+//  * the top quotient digit is left unmasked too for 2 and 3 limbs (all
+//    digits: m < 4 R (1 + 2^-30)), so the accumulators grow to a < 5 n:
+//    (a b + m n) / R < 5 n (n/4) / R + 4 n (1 + 2^-30) < 5 n/8 + 4.01 n;
+//    4 limbs keep the top two digits masked (m < R (1 + 2^-28): a < 2 n);
+//  * the first output limb keeps all 32 bits of its column: the column sum
+//    S = lo + 2^32 hi is the limb lo plus a carry of 4 hi into the next
+//    column (one MAD instead of a mask and a 64-bit shift).
+// A lazy value is sum v[i] 2^(30 i) with v[0] < 2^32 and v[L-1] < 2^32; it
+// is only multiplied by record factors (b < n/4, 30-bit limbs) and brought
+// back to the strict form (< 2n, 30-bit limbs) by lazy_norm. Every column
+// stays below 2^64 for the band limits (2 p^2 < R, n's top limb below 2^29):
+// tests/test_arith_host.py computes the bounds limb by limb.
+template <int L>
+constexpr int lazy_unmasked_m() { return L <= 3 ? L : 2; }
+template <int L>
+constexpr uint32_t lazy_bound() { return L <= 3 ? 5 : 2; }  // a, r < lazy_bound * n
+
+template <int L, int K>
+CLW_HD void montmul_lazy(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, const Res<L> (&n)[K],
+                         const uint32_t (&ninv)[K]) {
+  uint32_t m[K][L];
+  uint64_t c[K];
+  uint32_t four = 4;
+  CLW_PIN_S(four);  // an opaque 4: the carry stays one MAD (4 hi would become a 64-bit shift of {hi, 0})
+#pragma unroll
+  for (int j = 0; j < K; ++j) c[j] = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * L - 1; ++k) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int jj = k - i;
+      if (jj >= 0 && jj < L) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) c[j] = mad32(a[j].v[i], b.v[jj], c[j]);
+        pin_all<K>(c);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int jj = k - i;
+      if (i < k && jj >= 0 && jj < L) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) c[j] = mad32(m[j][i], n[j].v[jj], c[j]);
+        pin_all<K>(c);
+      }
+    }
+    if (k < L) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint32_t mk = (uint32_t)c[j] * ninv[j];
+        m[j][k] = k < lazy_unmasked_m<L>() ? mk : (mk & LMASK);
+        c[j] = mad32(m[j][k], n[j].v[0], c[j]) >> LB;
+      }
+      pin_all<K>(c);
+    } else if (k == L) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        r[j].v[0] = (uint32_t)c[j];
+        uint32_t hi = (uint32_t)(c[j] >> 32);
+        CLW_PIN32(hi);  // else 4 hi is rebuilt as (c >> 30) & ~3: three instructions
+        c[j] = mad32(hi, four, 0);
+      }
+      pin_all<K>(c);
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        r[j].v[k - L] = (uint32_t)c[j] & LMASK;
+        c[j] >>= LB;
+      }
+      pin_all<K>(c);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) r[j].v[L - 1] = (uint32_t)c[j];
+}
+
+// Strict form (< 2n, 30-bit limbs) of a lazy value; same residue mod n.
+// 32-bit limb arithmetic only (a 128-bit version raises the kernel's
+// register peak, which sits in the run flush).
+template <int L>
+CLW_HD Res<L> lazy_norm(const Res<L>& x, const Res<L>& n) {
+  Res<L> y;
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const uint32_t t = x.v[i] + carry;  // x.v[i] < 2^32 only for i = 0 and the top limb
+    y.v[i] = i < L - 1 ? (t & LMASK) : t;
+    carry = i == 0 ? (x.v[0] >> LB) : (t >> LB);
+  }
+  // y < lazy_bound n with 30-bit limbs below the top one: subtract n while
+  // y >= 2 n (at most lazy_bound - 2 times)
+#pragma unroll
+  for (uint32_t s = 2; s < lazy_bound<L>(); ++s) {
+    Res<L> d, d2;
+    uint32_t b1 = 0, b2 = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {  // d = y - n, d2 = y - 2n
+      const uint32_t s1 = n.v[i] + b1;
+      d.v[i] = i < L - 1 ? ((y.v[i] - s1) & LMASK) : (y.v[i] - s1);
+      b1 = y.v[i] < s1;
+      const uint32_t s2 = n.v[i] + b2;
+      d2.v[i] = i < L - 1 ? ((d.v[i] - s2) & LMASK) : (d.v[i] - s2);
+      b2 = d.v[i] < s2;
+    }
+    if (!b1 && !b2) y = d;  // y >= 2 n
+  }
+  return y;
 }
 
 // r - n if r >= n, else r (limb-wise borrow).

@@ -29,6 +29,27 @@ static void montmul_lockstep_t(const uint64_t a[2], const uint64_t b[2], const u
   out[1] = (uint64_t)(R >> 64);
 }
 
+// Lazy product on raw limbs (the lazy form is not a plain 128-bit value:
+// limb 0 may hold 32 bits).
+template <int L>
+static void montmul_lazy_t(const uint32_t* a, const uint64_t b[2], const uint64_t n[2], uint32_t* out) {
+  const u128 B = ((u128)b[1] << 64) | b[0], Nv = ((u128)n[1] << 64) | n[0];
+  Res<L> ra[1], rn[1] = {res_from_u128<L>(Nv)}, r[1];
+  for (int i = 0; i < L; ++i) ra[0].v[i] = a[i];
+  const uint32_t ninv[1] = {neg_inv30(rn[0].v[0])};
+  montmul_lazy<L, 1>(r, ra, res_from_u128<L>(B), rn, ninv);
+  for (int i = 0; i < L; ++i) out[i] = r[0].v[i];
+}
+
+template <int L>
+static void lazy_norm_t(const uint32_t* x, const uint64_t n[2], uint32_t* out) {
+  const u128 Nv = ((u128)n[1] << 64) | n[0];
+  Res<L> rx;
+  for (int i = 0; i < L; ++i) rx.v[i] = x[i];
+  const Res<L> y = lazy_norm<L>(rx, res_from_u128<L>(Nv));
+  for (int i = 0; i < L; ++i) out[i] = y.v[i];
+}
+
 template <int L>
 static Ctx<L> make_ctx(uint64_t p) {
   const u128 N = (u128)p * p;
@@ -71,6 +92,34 @@ int clwh_montmul_lockstep(int L, const uint64_t* a, const uint64_t* b, const uin
     case 3: montmul_lockstep_t<3>(a, b, n, out); return 0;
     case 4: montmul_lockstep_t<4>(a, b, n, out); return 0;
     case 5: montmul_lockstep_t<5>(a, b, n, out); return 0;
+  }
+  return -1;
+}
+
+int clwh_montmul_lazy(int L, const uint32_t* a, const uint64_t* b, const uint64_t* n, uint32_t* out) {
+  switch (L) {
+    case 2: montmul_lazy_t<2>(a, b, n, out); return 0;
+    case 3: montmul_lazy_t<3>(a, b, n, out); return 0;
+    case 4: montmul_lazy_t<4>(a, b, n, out); return 0;
+  }
+  return -1;
+}
+
+int clwh_lazy_norm(int L, const uint32_t* x, const uint64_t* n, uint32_t* out) {
+  switch (L) {
+    case 2: lazy_norm_t<2>(x, n, out); return 0;
+    case 3: lazy_norm_t<3>(x, n, out); return 0;
+    case 4: lazy_norm_t<4>(x, n, out); return 0;
+  }
+  return -1;
+}
+
+// (unmasked quotient digits, bound B of the lazy form a < B n)
+int clwh_lazy_params(int L, int* bound) {
+  switch (L) {
+    case 2: *bound = (int)lazy_bound<2>(); return lazy_unmasked_m<2>();
+    case 3: *bound = (int)lazy_bound<3>(); return lazy_unmasked_m<3>();
+    case 4: *bound = (int)lazy_bound<4>(); return lazy_unmasked_m<4>();
   }
   return -1;
 }

@@ -25,6 +25,10 @@ def hc():
     L.clwh_undrift.argtypes = [C.c_int, C.c_uint64, u64p, C.c_uint64, u64p]
     L.clwh_montmul_lockstep.argtypes = [C.c_int, u64p, u64p, u64p, u64p]
     L.clwh_unmasked_m.argtypes = [C.c_int]
+    u32p = C.POINTER(C.c_uint32)
+    L.clwh_montmul_lazy.argtypes = [C.c_int, u32p, u64p, u64p, u32p]
+    L.clwh_lazy_norm.argtypes = [C.c_int, u32p, u64p, u32p]
+    L.clwh_lazy_params.argtypes = [C.c_int, C.POINTER(C.c_int)]
     L.clwh_neg_inv30.restype = C.c_uint32
     L.clwh_neg_inv30.argtypes = [C.c_uint32]
     L.clwh_legendre.restype = C.c_uint64
@@ -119,6 +123,107 @@ def test_montmul_lockstep_bounds(hc, L):
             hc.clwh_montmul_lockstep(L, _arr(a), _arr(b), _arr(N), out)
             r = out[0] | (out[1] << 64)
             assert r < 2 * N and r % N == a * b * pow(R, -1, N) % N, (L, p, a, b)
+
+
+# Largest p of each batch limb band (2 p^2 < R = 2^(30 L)).
+BAND_PMAX = {2: 759250124, 3: 24879108095803, 4: 815238614083298888}
+
+
+def _lazy_params(hc, L):
+    b = C.c_int()
+    u = hc.clwh_lazy_params(L, C.byref(b))
+    return u, b.value
+
+
+def _lazy_limbs(v, L, t):
+    """Lazy-form limbs of value v: limb 0 takes t extra units of 2^30 (so up
+    to 32 bits), the top limb takes what is left."""
+    lo = (v % 2**30) + 2**30 * t
+    assert lo < 2**32 and lo <= v
+    rest = (v - lo) >> 30
+    limbs = [lo]
+    for i in range(1, L):
+        limbs.append(rest if i == L - 1 else rest % 2**30)
+        rest >>= 30
+    assert limbs[-1] < 2**32
+    return limbs
+
+
+def _limbs_value(x):
+    return sum(int(w) << (30 * i) for i, w in enumerate(x))
+
+
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_lazy_column_bounds(hc, L):
+    """Every 64-bit column sum of montmul_lazy fits, limb by limb, for the
+    largest modulus of the band: a in lazy form (limb 0 below 2^32, value
+    below B N), b < N/4, N's top limb from the band limit, unmasked quotient
+    digits below 2^32; and the lazy bound a < B N is preserved:
+    B N^2 / (4 R) + m N / R < B N."""
+    U, B = _lazy_params(hc, L)
+    R = 2 ** (30 * L)
+    N = BAND_PMAX[L] ** 2
+    assert 2 * N < R
+    top = 30 * (L - 1)
+    amax = [2**32 - 1] + [2**30 - 1] * (L - 2) + [min(2**32 - 1, (B * N) >> top)]
+    bmax = [2**30 - 1] * (L - 1) + [(N // 4) >> top]
+    nmax = [2**30 - 1] * (L - 1) + [N >> top]
+    mmax = [2**32 - 1 if k < U else 2**30 - 1 for k in range(L)]
+    carry = worst = 0
+    for k in range(2 * L - 1):
+        s = carry + sum(amax[i] * bmax[k - i] for i in range(L) if 0 <= k - i < L)
+        s += sum(mmax[i] * nmax[k - i] for i in range(L) if i < k and 0 <= k - i < L)
+        if k < L:
+            s += mmax[k] * nmax[0]
+        worst = max(worst, s)
+        carry = 4 * (s >> 32) if k == L else s >> 30
+    assert worst < 2**64, (L, worst / 2**64)
+    m_total = sum(mk << (30 * k) for k, mk in enumerate(mmax))
+    assert B * N * N // 4 + m_total * N < B * N * R
+    # the top limb of a result below B N fits 32 bits
+    assert (B * N) >> top < 2**32
+
+
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_montmul_lazy_products_and_norm(hc, L):
+    """The lazy product on extreme and random lazy operands is a b R^-1 mod
+    N, below B N, with limb 0 below 2^32 and the middle limbs below 2^30;
+    chains of products stay exact; lazy_norm gives the strict form (30-bit
+    limbs, below 2N) of the same residue."""
+    U, B = _lazy_params(hc, L)
+    R = 2 ** (30 * L)
+    Rinv_cache = {}
+    rng = random.Random(500 + L)
+    pmax = BAND_PMAX[L]
+    primes = [sympy.prevprime(pmax)] + [sympy.nextprime(rng.randrange(pmax // 4, pmax)) for _ in range(25)]
+    a_buf, out = (C.c_uint32 * L)(), (C.c_uint32 * L)()
+    for p in primes:
+        N = p * p
+        Rinv = pow(R, -1, N)
+        vals = [B * N - 1, B * N - 2**31, 1, 2**32 - 1] + [rng.randrange(B * N) for _ in range(40)]
+        for v in vals:
+            tmax = min(3, (v >> 30), (2**32 - 1 - v % 2**30) >> 30)
+            a = _lazy_limbs(v, L, rng.randint(0, tmax) if tmax > 0 else 0)
+            for b in (N // 4 - 1, 1, rng.randrange(N // 4)):
+                a_buf[:] = a
+                hc.clwh_montmul_lazy(L, a_buf, _arr(b), _arr(N), out)
+                r = _limbs_value(out)
+                assert r % N == v * b * Rinv % N and r < B * N, (L, p, v, b)
+                assert out[0] < 2**32 and all(out[i] < 2**30 for i in range(1, L - 1))
+                hc.clwh_lazy_norm(L, out, _arr(N), a_buf)
+                s = _limbs_value(a_buf)
+                assert s < 2 * N and s % N == r % N and all(w < 2**30 for w in a_buf)
+        # a chain of 300 products from the largest lazy value
+        x = _lazy_limbs(B * N - 1, L, 0)
+        want = (B * N - 1) % N
+        for _ in range(300):
+            b = rng.choice([N // 4 - 1, rng.randrange(N // 4)])
+            a_buf[:] = x
+            hc.clwh_montmul_lazy(L, a_buf, _arr(b), _arr(N), out)
+            x = list(out)
+            want = want * b * Rinv % N
+            assert _limbs_value(x) < B * N
+        assert _limbs_value(x) % N == want
 
 
 @pytest.mark.parametrize("L", [2, 3, 4, 5])
