@@ -459,6 +459,7 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
         for (uint32_t b = 0; b < nblk_t[t]; ++b) recs += nrec_h[b], prs += npr_h[b];
         S->ks.mul_record_apps += recs * S->cnt[t];
         S->ks.mul_prime_apps += prs * S->cnt[t];
+        S->ks.mul_partial_products += recs * S->cnt[t] * (2 * S->tab.limbs * S->tab.limbs + S->tab.limbs);
       }
       S->ks.side_launches++;
       S->ks.side_ms += ev_ms(e4[1], e4[2]);

@@ -252,6 +252,77 @@ CLW_HD void montmul_lazy(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, 
   for (int j = 0; j < K; ++j) r[j].v[L - 1] = (uint32_t)c[j];
 }
 
+// Long Montgomery reduction of a D-digit integer P (30-bit digits, D a
+// multiple of L, wave-uniform: the product of a run of sieving primes) for K
+// moduli in lockstep: y[j] = P 2^(-30 D) mod n[j], y[j] < 2 n[j] (strict
+// form). Column i (weight 2^(30 i)) receives p_i, the carry and m_{i-d} n_d
+// for d = 1..L-1, then m_i = c * ninv mod 2^32 zeroes it mod 2^30 with
+// m_i n_0; per digit: L MADs, one mul_lo, the digit add (a MAD by an opaque
+// 1) and one 64-bit shift. The quotient digits of the last L columns are
+// masked, so M = sum m_i 2^(30 i) < 2^(30 D) (1 + 2^-28) and
+// y = (P + M n) / 2^(30 D) < 1 + n (1 + 2^-28) < 2 n. Column sums stay below
+// 2^35 + 2^30 + (L - 1) 2^62 + 2^62 < 2^64 for L <= 3; L = 4 adds one more
+// 2^62 term (n's top limb is below 2^29, so that product is below 2^61).
+// One group of L digits of redc_long (MASKED: the last group).
+template <int L, int K, bool MASKED>
+CLW_HD void redc_group(uint64_t (&c)[K], uint32_t (&m)[K][L], const uint32_t* __restrict__ pg, uint32_t one,
+                       const Res<L> (&n)[K], const uint32_t (&ninv)[K]) {
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const uint32_t pd = pg[t];
+#pragma unroll
+    for (int j = 0; j < K; ++j) c[j] = mad32(pd, one, c[j]);
+    pin_all<K>(c);
+#pragma unroll
+    for (int d = 1; d < L; ++d) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) c[j] = mad32(m[j][(t - d + L) % L], n[j].v[d], c[j]);
+      pin_all<K>(c);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t mk = (uint32_t)c[j] * ninv[j];
+      m[j][t] = MASKED ? (mk & LMASK) : mk;
+      c[j] = mad32(m[j][t], n[j].v[0], c[j]) >> LB;
+    }
+    pin_all<K>(c);
+  }
+}
+
+template <int L, int K>
+CLW_HD void redc_long(Res<L> (&y)[K], const uint32_t* __restrict__ p, uint32_t D, const Res<L> (&n)[K],
+                      const uint32_t (&ninv)[K]) {
+  uint32_t m[K][L];
+  uint64_t c[K];
+  uint32_t one = 1;
+  CLW_PIN_S(one);  // an opaque 1: the digit add stays one MAD (64-bit add + carry would be two)
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    c[j] = 0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) m[j][t] = 0;
+  }
+  const uint32_t groups = D / L;  // >= 1
+  for (uint32_t g = 0; g + 1 < groups; ++g) redc_group<L, K, false>(c, m, p + g * L, one, n, ninv);
+  redc_group<L, K, true>(c, m, p + (groups - 1) * L, one, n, ninv);
+  // columns D .. D + L - 1: the remaining m_{D-1-s} n_d, then the carry
+#pragma unroll
+  for (int k = 0; k < L - 1; ++k) {
+#pragma unroll
+    for (int d = k + 1; d < L; ++d) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) c[j] = mad32(m[j][(L + k - d) % L], n[j].v[d], c[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      y[j].v[k] = (uint32_t)c[j] & LMASK;
+      c[j] >>= LB;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) y[j].v[L - 1] = (uint32_t)c[j];
+}
+
 // Strict form (< 2n, 30-bit limbs) of a lazy value; same residue mod n.
 // 32-bit limb arithmetic only (a 128-bit version raises the kernel's
 // register peak, which sits in the run flush).

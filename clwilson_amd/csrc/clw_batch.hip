@@ -48,6 +48,8 @@ const double kAppsPerSegment = 2e12;   // factor applications aimed at per segme
 const uint32_t kItemsPerBlock = 32;    // work items per resident block (balance)
 const uint32_t kMaxChunks = 256;
 const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
+const size_t kTotalsBytes = 64;            // device counters (8 words)
+const uint32_t LB_DIGIT = 30;              // bits per digit of a run product
 
 inline uint32_t prime_type(uint64_t p) { return p % 3 == 1 ? 0 : (p % 12 == 5 ? 1 : 2); }
 inline uint64_t type_divisor(uint32_t t) { return t == 0 ? 6 : (t == 1 ? 4 : 2); }
@@ -129,7 +131,11 @@ struct clw_batch {
   uint32_t* rec = nullptr;
   uint32_t* nrec = nullptr;
   uint32_t* flags = nullptr;
-  uint64_t* totals = nullptr;  // [0] this segment, [1] accumulated
+  uint64_t* totals = nullptr;  // primes: [0] this segment, [1] accumulated; products: [2] records,
+                               // [3] primes, [4] 30x30-bit partial products
+  uint32_t* subp = nullptr;    // run products: digits of each (block, sub-chunk)
+  uint32_t* subn = nullptr;    // their digit counts
+  uint32_t ds = 0;             // digit stride (a multiple of 12)
   SegHost* seg_d = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -167,7 +173,7 @@ struct clw_batch {
         clw_free(ctx, b.part_hi);
       }
     }
-    void* ptrs[] = {rec, nrec, flags, totals, seg_d};
+    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn};
     for (void* x : ptrs) clw_free(ctx, x);
   }
 };
@@ -278,10 +284,18 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   int rc = balloc((size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * 4, (void**)&B->nrec);
   if (rc == CLW_OK) rc = balloc(16, (void**)&B->flags);
-  if (rc == CLW_OK) rc = balloc(32, (void**)&B->totals);
+  if (rc == CLW_OK) rc = balloc(kTotalsBytes, (void**)&B->totals);
   if (rc == CLW_OK) rc = balloc(sizeof(SegHost), (void**)&B->seg_d);
+  // run products: 64 sieving primes below maxT + 1 take at most 64 bits(maxT) bits
+  {
+    const uint32_t bits = B->maxT ? 64 - __builtin_clzll(B->maxT) : 1;
+    const uint32_t digits = (64 * bits + LB_DIGIT - 1) / LB_DIGIT;
+    B->ds = std::min<uint32_t>(kBatchSubMaxDigits, std::max<uint32_t>(12, (digits + 11) / 12 * 12));
+  }
+  if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);
+  if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subn);
   if (rc == CLW_OK) rc = clw_memset(ctx, B->flags, 0, 16);
-  if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, 32);
+  if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, kTotalsBytes);
   for (Band& bd : B->bands) {
     if (!bd.L || rc != CLW_OK) continue;
     bd.per_lane = B->opt.per_lane ? B->opt.per_lane : clw_batch_default_per_lane((uint32_t)bd.L);
@@ -460,13 +474,15 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   CK(mark(B, timed));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+  CK(clwi_batch_subprod(ctx, B->seg_d, extra_block, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
+                        B->flags));
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
     Band& bd = B->bands[band];
     if (!bd.L) continue;
     CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
-                      B->seg_d, (uint32_t)(band - 2), extra_block, bd.resident, bd.part_lo, bd.part_hi,
-                      B->totals + 2));
+                      B->seg_d, (uint32_t)(band - 2), extra_block, B->subp, B->subn, B->ds, bd.resident,
+                      bd.part_lo, bd.part_hi, B->totals + 2));
   }
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
@@ -591,16 +607,17 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   clw_ctx* ctx = B->ctx;
   const uint64_t qlo = B->pending.qlo, qhi = B->pending.qhi;
   uint32_t flags[4] = {0, 0, 0, 0};
-  uint64_t tot[4] = {0, 0, 0, 0};
+  uint64_t tot[5] = {0, 0, 0, 0, 0};
   CK(clw_d2h(ctx, flags, B->flags, 16, 0));
-  CK(clw_d2h(ctx, tot, B->totals, 32, 1));
+  CK(clw_d2h(ctx, tot, B->totals, sizeof(tot), 1));
   if (flags[0]) return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded: flags %u", flags[0]);
   uint64_t smallc = 0;
   for (uint64_t q : {2ull, 3ull, 5ull}) smallc += (q >= qlo && q < qhi);
   B->applied += tot[1] + smallc;
   B->ks.mul_record_apps += tot[2];
   B->ks.mul_prime_apps += tot[3];
-  CK(clw_memset(ctx, B->totals, 0, 32));
+  B->ks.mul_partial_products += tot[4];
+  CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   for (uint32_t k = 0; k + 4 <= B->ev_used; k += 4) {
     float a = 0, b = 0, c = 0;
     CK(clw_hip(hipEventElapsedTime(&a, B->events[k], B->events[k + 1]), "hipEventElapsedTime"));
@@ -837,25 +854,27 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   h.span = span_blocks * CLW_BLOCK_SPAN;
   h.end = q + h.span;
   CK(clw_h2d(ctx, B->seg_d, &h, sizeof(h), 1));
-  CK(clw_memset(ctx, B->totals, 0, 32));
+  CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds, B->flags));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = clw_hip(hipEventCreate(&e0), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventCreate(&e1), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e0, clw_stream(ctx)), "hipEventRecord");
   if (rc == CLW_OK)
     rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
-                        (uint32_t)(bd.L - 2), 0, bd.resident, bd.part_lo, bd.part_hi, B->totals + 2);
+                        (uint32_t)(bd.L - 2), 0, B->subp, B->subn, B->ds, bd.resident, bd.part_lo, bd.part_hi,
+                        B->totals + 2);
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e1, clw_stream(ctx)), "hipEventRecord");
-  uint64_t tot[4] = {0, 0, 0, 0};
-  if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, 32, 1);
+  uint64_t tot[5] = {0, 0, 0, 0, 0};
+  if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, sizeof(tot), 1);
   float ms = 0;
   if (rc == CLW_OK) rc = clw_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   CK(rc);
-  CK(clw_memset(ctx, B->totals, 0, 32));
+  CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   *rate = ms > 0 ? (double)tot[3] / (ms * 1e-3) : 0.0;
   return CLW_OK;
 }

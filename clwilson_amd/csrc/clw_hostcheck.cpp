@@ -42,6 +42,22 @@ static void montmul_lazy_t(const uint32_t* a, const uint64_t b[2], const uint64_
 }
 
 template <int L>
+static void redc_long_t(const uint32_t* pd, uint32_t D, const uint64_t n[2], uint64_t out[2]) {
+  const u128 Nv = ((u128)n[1] << 64) | n[0];
+  Res<L> rn[1] = {res_from_u128<L>(Nv)}, y[1];
+  const uint32_t ninv[1] = {neg_inv30(rn[0].v[0])};
+  redc_long<L, 1>(y, pd, D, rn, ninv);
+  for (int i = 0; i < L; ++i)
+    if (y[0].v[i] >> LB) {  // limbs must be 30-bit
+      out[0] = out[1] = ~0ull;
+      return;
+    }
+  const u128 R = res_to_u128<L>(y[0]);
+  out[0] = (uint64_t)R;
+  out[1] = (uint64_t)(R >> 64);
+}
+
+template <int L>
 static void lazy_norm_t(const uint32_t* x, const uint64_t n[2], uint32_t* out) {
   const u128 Nv = ((u128)n[1] << 64) | n[0];
   Res<L> rx;
@@ -101,6 +117,16 @@ int clwh_montmul_lazy(int L, const uint32_t* a, const uint64_t* b, const uint64_
     case 2: montmul_lazy_t<2>(a, b, n, out); return 0;
     case 3: montmul_lazy_t<3>(a, b, n, out); return 0;
     case 4: montmul_lazy_t<4>(a, b, n, out); return 0;
+  }
+  return -1;
+}
+
+int clwh_redc_long(int L, const uint32_t* p, uint32_t D, const uint64_t* n, uint64_t* out) {
+  if (D == 0 || D % (uint32_t)L) return -1;
+  switch (L) {
+    case 2: redc_long_t<2>(p, D, n, out); return 0;
+    case 3: redc_long_t<3>(p, D, n, out); return 0;
+    case 4: redc_long_t<4>(p, D, n, out); return 0;
   }
   return -1;
 }

@@ -27,6 +27,13 @@ static inline int clw_hip(hipError_t e, const char* what) { return e == hipSucce
 static inline hipStream_t clw_stream(clw_ctx* ctx) { return ctx->stream; }
 int clw_upload_sieve_tables();
 
+// Run products of the batched engine: a sub-chunk is kBatchSubRecs pair
+// records (64 sieving primes), at most kBatchSubCap per sieve block, each
+// multiplied out into at most kBatchSubMaxDigits 30-bit digits.
+constexpr uint32_t kBatchSubRecs = 32;
+constexpr uint32_t kBatchSubCap = CLW_REC_CAP / kBatchSubRecs;
+constexpr uint32_t kBatchSubMaxDigits = 144;
+
 // Batched-engine launchers (clw_kernels.hip) used by clw_batch.hip.
 extern "C" {
 int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uint32_t max_blocks, uint32_t* flags,
@@ -34,8 +41,10 @@ int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uin
 int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_lane, uint32_t* out);
 int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
                    const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec, void* seg,
-                   uint32_t qslot, uint32_t extra_block, uint32_t grid, uint64_t* part_lo, uint64_t* part_hi,
-                   uint64_t* apps);
+                   uint32_t qslot, uint32_t extra_block, const uint32_t* subp, const uint32_t* subn, uint32_t ds,
+                   uint32_t grid, uint64_t* part_lo, uint64_t* part_hi, uint64_t* apps);
+int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec, const uint32_t* nrec,
+                       uint32_t max_blocks, uint32_t* subp, uint32_t* subn, uint32_t ds, uint32_t* flags);
 int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
                       const void* plan, const uint64_t* part_lo, const uint64_t* part_hi);
 int clwi_seg_advance(clw_ctx* ctx, void* seg, uint32_t* flags, uint64_t* total_prev, uint64_t* total);

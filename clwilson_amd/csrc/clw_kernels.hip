@@ -1107,14 +1107,27 @@ extern "C" int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_la
 // the largest plan); the kernel reads its plan from `plan`.
 extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
                               const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec,
-                              void* seg, uint32_t qslot, uint32_t extra_block, uint32_t grid, uint64_t* part_lo,
-                              uint64_t* part_hi, uint64_t* apps) {
+                              void* seg, uint32_t qslot, uint32_t extra_block, const uint32_t* subp,
+                              const uint32_t* subn, uint32_t ds, uint32_t grid, uint64_t* part_lo, uint64_t* part_hi,
+                              uint64_t* apps) {
   if (grid == 0) return CLW_OK;
   DISPATCH_BK(t->limbs, per_lane,
               hipLaunchKernelGGL((k_batch_mul<LL, KK>), dim3(grid), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
                                  (const BatchGroup*)groups, item_gt, (const BatchBand*)plan, rec, nrec,
-                                 (BatchSeg*)seg, qslot, extra_block, part_lo, part_hi, (unsigned long long*)apps));
+                                 (BatchSeg*)seg, qslot, extra_block, subp, subn, ds, part_lo, part_hi,
+                                 (unsigned long long*)apps));
   return clw_check_launch("batch_mul");
+}
+
+// One thread per (block, sub-chunk) slot of the record buffer.
+extern "C" int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec,
+                                  const uint32_t* nrec, uint32_t max_blocks, uint32_t* subp, uint32_t* subn,
+                                  uint32_t ds, uint32_t* flags) {
+  if (ds % 12 || ds > kBatchSubMaxDigits) return clw_fail(CLW_ERR_ARG, "run products: bad digit stride %u", ds);
+  const uint32_t threads = max_blocks * kBatchSubCap;
+  hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
+                     (const BatchSeg*)seg, extra_block, rec, nrec, subp, subn, ds, flags);
+  return clw_check_launch("subchunk_prod");
 }
 
 extern "C" int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,

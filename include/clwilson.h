@@ -280,13 +280,17 @@ int clw_session_finalize(clw_session* s, clw_range_stats* stats, uint64_t* out_p
 
 /* Per-kernel timing of a session, from HIP events recorded around every
  * launch of clw_session_run (no counterpart in the reference beyond
- * ProfilesclEnqueueKernel). record_apps = records x test primes applied. */
+ * ProfilesclEnqueueKernel). record_apps = full Montgomery products x test
+ * primes; prime_apps = sieving primes x test primes applied;
+ * partial_products = the 30x30-bit partial products of the product kernels
+ * (2L^2 + L per full product, L + 1 per digit of a run product reduced). */
 typedef struct {
   uint64_t segments;
   uint64_t sieve_launches, sieve_integers;
   uint64_t mul_launches, mul_record_apps, mul_prime_apps;
   uint64_t side_launches, reduce_launches;
   double sieve_ms, mul_ms, side_ms, reduce_ms;
+  uint64_t mul_partial_products;
 } clw_kernel_stats;
 int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
 int clw_session_reset_stats(clw_session* s);

@@ -29,6 +29,7 @@ def hc():
     L.clwh_montmul_lazy.argtypes = [C.c_int, u32p, u64p, u64p, u32p]
     L.clwh_lazy_norm.argtypes = [C.c_int, u32p, u64p, u32p]
     L.clwh_lazy_params.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    L.clwh_redc_long.argtypes = [C.c_int, u32p, C.c_uint32, u64p, u64p]
     L.clwh_neg_inv30.restype = C.c_uint32
     L.clwh_neg_inv30.argtypes = [C.c_uint32]
     L.clwh_legendre.restype = C.c_uint64
@@ -224,6 +225,36 @@ def test_montmul_lazy_products_and_norm(hc, L):
             want = want * b * Rinv % N
             assert _limbs_value(x) < B * N
         assert _limbs_value(x) % N == want
+
+
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_redc_long(hc, L):
+    """Long Montgomery reduction of a D-digit integer (the product of a run
+    of sieving primes): P 2^(-30 D) mod N in strict form (30-bit limbs,
+    below 2N), for all-ones digits, products of real primes and random
+    digits, D from L to 150 digits."""
+    rng = random.Random(700 + L)
+    pmax = BAND_PMAX[L]
+    primes = [sympy.prevprime(pmax)] + [sympy.nextprime(rng.randrange(pmax // 4, pmax)) for _ in range(12)]
+    out = (C.c_uint64 * 2)()
+    for p in primes:
+        N = p * p
+        for D in sorted({L, 2 * L, 30 - 30 % L, 96 - 96 % L, 150 - 150 % L}):
+            cases = [[2**30 - 1] * D, [0] * (D - 1) + [1], [rng.randrange(2**30) for _ in range(D)]]
+            # a product of consecutive primes (the run products the kernel reduces)
+            q0 = sympy.nextprime(rng.randrange(2, min(p, 2**40)))
+            P, q = 1, q0
+            while P.bit_length() + q.bit_length() <= 30 * D:
+                P *= q
+                q = sympy.nextprime(q)
+            cases.append([(P >> (30 * i)) & (2**30 - 1) for i in range(D)])
+            for digits in cases:
+                P = sum(d << (30 * i) for i, d in enumerate(digits))
+                arr = (C.c_uint32 * D)(*digits)
+                assert hc.clwh_redc_long(L, arr, D, _arr(N), out) == 0
+                y = out[0] | (out[1] << 64)
+                assert y < 2 * N, (L, p, D)
+                assert y % N == P * pow(2, -30 * D, N) % N, (L, p, D)
 
 
 @pytest.mark.parametrize("L", [2, 3, 4, 5])
