@@ -330,7 +330,8 @@ def profile_run(cw, args) -> int:
     run.close()
     dev.close()
     print(json.dumps(dict(copy_bytes=n, kernel=run.kernel, launches=ks["mul_launches"],
-                          mul_ms=ks["mul_ms"], record_apps=ks["mul_record_apps"])))
+                          mul_ms=ks["mul_ms"], record_apps=ks["mul_record_apps"],
+                          prime_apps=ks["mul_prime_apps"], partial_products=ks["mul_partial_products"])))
     return 0
 
 
@@ -435,7 +436,10 @@ def main() -> int:
 
     peak = info["compute_units"] * 64 * info["clock_khz"] * 1e3  # v_mad_u64_u32 / s
     prod_per_rec = 2 * L * L + L
-    achieved = ks["mul_record_apps"] * prod_per_rec / (ks["mul_ms"] * 1e-3) if ks["mul_ms"] else 0.0
+    # algorithmic 30x30-bit partial products counted by the product kernel:
+    # 2L^2 + L per full Montgomery product (records, one per run product),
+    # L + 1 per 30-bit digit of a run product reduced mod p^2
+    achieved = ks["mul_partial_products"] / (ks["mul_ms"] * 1e-3) if ks["mul_ms"] else 0.0
     batch256 = measure_batch256(dev, cw, args.batch_steps, not args.no_tune) if args.batch_steps > 0 else None
     serving = measure_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     out = {
@@ -476,7 +480,10 @@ def main() -> int:
                         + BATCH_MUL_TRAFFIC["write_kib"] * BATCH_MUL_TRAFFIC["write_scale"]) * 1024
             if run.kernel == "k_batch_mul<3,3>" and args.range is None and not args.tune_single else None,
             "traffic_of": BATCH_MUL_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
-            "per_unit": f"{prod_per_rec} 30x30-bit partial products per record per test prime",
+            "per_unit": f"30x30-bit partial products per test prime: {L + 1} per 30-bit digit of a run product "
+                        f"reduced, {prod_per_rec} per full Montgomery product (records across run boundaries, "
+                        "one per run product)",
+            "partial_products": ks["mul_partial_products"],
             "launches": ks["mul_launches"],
             "mean_launch_ms": ks["mul_ms"] / max(1, ks["mul_launches"]),
         },

@@ -286,10 +286,11 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   if (rc == CLW_OK) rc = balloc(16, (void**)&B->flags);
   if (rc == CLW_OK) rc = balloc(kTotalsBytes, (void**)&B->totals);
   if (rc == CLW_OK) rc = balloc(sizeof(SegHost), (void**)&B->seg_d);
-  // run products: 64 sieving primes below maxT + 1 take at most 64 bits(maxT) bits
+  // run products: 2 kBatchSubRecs sieving primes below maxT + 1 take at most
+  // 2 kBatchSubRecs bits(maxT) bits
   {
     const uint32_t bits = B->maxT ? 64 - __builtin_clzll(B->maxT) : 1;
-    const uint32_t digits = (64 * bits + LB_DIGIT - 1) / LB_DIGIT;
+    const uint32_t digits = (2 * kBatchSubRecs * bits + LB_DIGIT - 1) / LB_DIGIT;
     B->ds = std::min<uint32_t>(kBatchSubMaxDigits, std::max<uint32_t>(12, (digits + 11) / 12 * 12));
   }
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);

@@ -28,11 +28,12 @@ static inline hipStream_t clw_stream(clw_ctx* ctx) { return ctx->stream; }
 int clw_upload_sieve_tables();
 
 // Run products of the batched engine: a sub-chunk is kBatchSubRecs pair
-// records (64 sieving primes), at most kBatchSubCap per sieve block, each
-// multiplied out into at most kBatchSubMaxDigits 30-bit digits.
-constexpr uint32_t kBatchSubRecs = 32;
+// records (128 sieving primes), at most kBatchSubCap per sieve block, each
+// multiplied out into at most kBatchSubMaxDigits 30-bit digits (128 primes
+// below 2^62: 265 digits, rounded up to a multiple of 12).
+constexpr uint32_t kBatchSubRecs = 64;
 constexpr uint32_t kBatchSubCap = CLW_REC_CAP / kBatchSubRecs;
-constexpr uint32_t kBatchSubMaxDigits = 144;
+constexpr uint32_t kBatchSubMaxDigits = 276;
 
 // Batched-engine launchers (clw_kernels.hip) used by clw_batch.hip.
 extern "C" {

@@ -182,8 +182,8 @@ class Server:
     def admit(self) -> int:
         """One admission round; returns the number of requests admitted."""
         # record buffer (4096 records of 32 B per block) and run products
-        # (128 sub-chunks of at most 144 digits + a count per block)
-        fixed = (self.segment_span // BLOCK_SPAN + 2) * (4096 * 32 + 128 * 145 * 4) + (1 << 20)
+        # (64 sub-chunks of at most 276 digits + a count per block)
+        fixed = (self.segment_span // BLOCK_SPAN + 2) * (4096 * 32 + 64 * 277 * 4) + (1 << 20)
         classes: dict[int, list[Request]] = {}
         est: dict[int, int] = {}
         primes: dict[int, float] = {}
