@@ -257,54 +257,65 @@ CLW_HD void montmul_lazy(Res<L> (&r)[K], const Res<L> (&a)[K], const Res<L>& b, 
 // moduli in lockstep: y[j] = P 2^(-30 D) mod n[j], y[j] < 2 n[j] (strict
 // form). Column i (weight 2^(30 i)) receives p_i, the carry and m_{i-d} n_d
 // for d = 1..L-1, then m_i = c * ninv mod 2^32 zeroes it mod 2^30 with
-// m_i n_0; per digit: L MADs, one mul_lo, the digit add (a MAD by an opaque
-// 1) and one 64-bit shift. The quotient digits of the last L columns are
+// m_i n_0; per digit: L MADs, one mul_lo, one 64-bit shift and half a 64-bit
+// add (digits enter in pairs). The quotient digits of the last L columns are
 // masked, so M = sum m_i 2^(30 i) < 2^(30 D) (1 + 2^-28) and
 // y = (P + M n) / 2^(30 D) < 1 + n (1 + 2^-28) < 2 n. Column sums stay below
-// 2^35 + 2^30 + (L - 1) 2^62 + 2^62 < 2^64 for L <= 3; L = 4 adds one more
+// 2^35 + 2^60 + (L - 1) 2^62 + 2^62 < 2^64 for L <= 3; L = 4 adds one more
 // 2^62 term (n's top limb is below 2^29, so that product is below 2^61).
-// One group of L digits of redc_long (MASKED: the last group).
+// Digits per unrolled group of redc_long: a multiple of L (the ring of
+// quotient digits) and of 2 (digits enter in pairs).
+template <int L>
+constexpr int redc_span() { return L % 2 ? 2 * L : L; }
+
+// One group of redc_span digits of redc_long (MASKED: the last group, whose
+// last L quotient digits are masked). Digits enter two at a time: column i
+// gets p_i + p_{i+1} 2^30 with one 64-bit add (p_{i+1} lands in column i + 1
+// after the shift; the low 30 bits that fix m_i are unchanged).
 template <int L, int K, bool MASKED>
-CLW_HD void redc_group(uint64_t (&c)[K], uint32_t (&m)[K][L], const uint32_t* __restrict__ pg, uint32_t one,
-                       const Res<L> (&n)[K], const uint32_t (&ninv)[K]) {
+CLW_HD void redc_group(uint64_t (&c)[K], uint32_t (&m)[K][L], const uint32_t* __restrict__ pg, const Res<L> (&n)[K],
+                       const uint32_t (&ninv)[K]) {
+  constexpr int G = redc_span<L>();
 #pragma unroll
-  for (int t = 0; t < L; ++t) {
-    const uint32_t pd = pg[t];
+  for (int t = 0; t < G; ++t) {
+    if (t % 2 == 0) {
+      const uint64_t pr = (uint64_t)pg[t] | ((uint64_t)pg[t + 1] << LB);
 #pragma unroll
-    for (int j = 0; j < K; ++j) c[j] = mad32(pd, one, c[j]);
-    pin_all<K>(c);
+      for (int j = 0; j < K; ++j) c[j] += pr;
+      pin_all<K>(c);
+    }
 #pragma unroll
     for (int d = 1; d < L; ++d) {
 #pragma unroll
-      for (int j = 0; j < K; ++j) c[j] = mad32(m[j][(t - d + L) % L], n[j].v[d], c[j]);
+      for (int j = 0; j < K; ++j) c[j] = mad32(m[j][((t - d) % L + L) % L], n[j].v[d], c[j]);
       pin_all<K>(c);
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const uint32_t mk = (uint32_t)c[j] * ninv[j];
-      m[j][t] = MASKED ? (mk & LMASK) : mk;
-      c[j] = mad32(m[j][t], n[j].v[0], c[j]) >> LB;
+      m[j][t % L] = (MASKED && t >= G - L) ? (mk & LMASK) : mk;
+      c[j] = mad32(m[j][t % L], n[j].v[0], c[j]) >> LB;
     }
     pin_all<K>(c);
   }
 }
 
+// D: a multiple of redc_span<L>().
 template <int L, int K>
 CLW_HD void redc_long(Res<L> (&y)[K], const uint32_t* __restrict__ p, uint32_t D, const Res<L> (&n)[K],
                       const uint32_t (&ninv)[K]) {
+  constexpr uint32_t G = redc_span<L>();
   uint32_t m[K][L];
   uint64_t c[K];
-  uint32_t one = 1;
-  CLW_PIN_S(one);  // an opaque 1: the digit add stays one MAD (64-bit add + carry would be two)
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     c[j] = 0;
 #pragma unroll
     for (int t = 0; t < L; ++t) m[j][t] = 0;
   }
-  const uint32_t groups = D / L;  // >= 1
-  for (uint32_t g = 0; g + 1 < groups; ++g) redc_group<L, K, false>(c, m, p + g * L, one, n, ninv);
-  redc_group<L, K, true>(c, m, p + (groups - 1) * L, one, n, ninv);
+  const uint32_t groups = D / G;  // >= 1
+  for (uint32_t g = 0; g + 1 < groups; ++g) redc_group<L, K, false>(c, m, p + g * G, n, ninv);
+  redc_group<L, K, true>(c, m, p + (groups - 1) * G, n, ninv);
   // columns D .. D + L - 1: the remaining m_{D-1-s} n_d, then the carry
 #pragma unroll
   for (int k = 0; k < L - 1; ++k) {

@@ -122,7 +122,7 @@ int clwh_montmul_lazy(int L, const uint32_t* a, const uint64_t* b, const uint64_
 }
 
 int clwh_redc_long(int L, const uint32_t* p, uint32_t D, const uint64_t* n, uint64_t* out) {
-  if (D == 0 || D % (uint32_t)L) return -1;
+  if (D == 0 || D % (uint32_t)(L % 2 ? 2 * L : L)) return -1;
   switch (L) {
     case 2: redc_long_t<2>(p, D, n, out); return 0;
     case 3: redc_long_t<3>(p, D, n, out); return 0;

@@ -232,14 +232,15 @@ def test_redc_long(hc, L):
     """Long Montgomery reduction of a D-digit integer (the product of a run
     of sieving primes): P 2^(-30 D) mod N in strict form (30-bit limbs,
     below 2N), for all-ones digits, products of real primes and random
-    digits, D from L to 150 digits."""
+    digits, D from one group to 270 digits."""
     rng = random.Random(700 + L)
     pmax = BAND_PMAX[L]
     primes = [sympy.prevprime(pmax)] + [sympy.nextprime(rng.randrange(pmax // 4, pmax)) for _ in range(12)]
     out = (C.c_uint64 * 2)()
     for p in primes:
         N = p * p
-        for D in sorted({L, 2 * L, 30 - 30 % L, 96 - 96 % L, 150 - 150 % L}):
+        G = 2 * L if L % 2 else L  # digits per unrolled group
+        for D in sorted({G, 2 * G, 30 - 30 % G, 96 - 96 % G, 270 - 270 % G}):
             cases = [[2**30 - 1] * D, [0] * (D - 1) + [1], [rng.randrange(2**30) for _ in range(D)]]
             # a product of consecutive primes (the run products the kernel reduces)
             q0 = sympy.nextprime(rng.randrange(2, min(p, 2**40)))
