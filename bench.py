@@ -54,7 +54,7 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB
 # for the 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
 # profiles/batch_mul_cfg3_hbm_counters.md.
-BATCH_MUL_TRAFFIC = dict(fetch_kib=1405734.4, write_kib=494926.7, fetch_scale=2.0, write_scale=1.0,
+BATCH_MUL_TRAFFIC = dict(fetch_kib=1014349.8, write_kib=496138.1, fetch_scale=2.0, write_scale=1.0,
                          window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
 
 
