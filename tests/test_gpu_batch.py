@@ -126,6 +126,25 @@ def test_batch_window_residues_all_bands(dev, window):
                 assert b.residues(r) == s.residues()
 
 
+def test_batch_run_products_carry_large_q(dev):
+    """Above sqrt(T) almost every sieving prime enters through a run product
+    (one full Montgomery product per 128-prime sub-chunk, the digits reduced
+    mod p^2), and the window residues equal the session path's, which applies
+    every record as a product."""
+    lo, hi = 2 * 10**13, 2 * 10**13 + 100_000
+    qlo, qhi = 10**12, 10**12 + 2 * 10**9
+    with cw.Batch(dev, [(lo, hi)]) as b, cw.Session(dev, lo, hi) as s:
+        b.run(qlo, qhi)
+        s.run(qlo, qhi)
+        ks = b.kernel_stats()
+        assert b.residues(0) == s.residues()
+    say("prime apps", ks["mul_prime_apps"], "full products", ks["mul_record_apps"],
+        "partial products", ks["mul_partial_products"])
+    assert ks["mul_prime_apps"] > 0
+    assert ks["mul_record_apps"] * 100 < ks["mul_prime_apps"]
+    assert ks["mul_partial_products"] < 7 * ks["mul_prime_apps"]  # ~5.9 per prime at this size
+
+
 def test_batch_rejects_bad_ranges(dev):
     with pytest.raises(cw.ClwError):
         cw.Batch(dev, [(10, 10)])
