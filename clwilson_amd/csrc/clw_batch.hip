@@ -276,7 +276,9 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       B->opt.segment_span ? (uint32_t)std::min<uint64_t>((B->opt.segment_span + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN,
                                                          kMaxSpanBlocks)
                           : kMaxSpanBlocks;
-  B->max_blocks = span_blocks + 2;  // + the slot for the primes 2, 3, 5
+  // no segment needs more blocks than cover [0, maxT]
+  const uint32_t need = (uint32_t)std::min<uint64_t>(kMaxSpanBlocks, B->maxT / CLW_BLOCK_SPAN + 2);
+  B->max_blocks = std::max<uint32_t>(kMinSpanBlocks, std::min(span_blocks, need)) + 2;  // + the slot for 2, 3, 5
   auto balloc = [&](size_t sz, void** ptr) {
     B->bytes += sz;
     return clw_alloc(ctx, sz, ptr);
@@ -287,9 +289,9 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   if (rc == CLW_OK) rc = balloc(kTotalsBytes, (void**)&B->totals);
   if (rc == CLW_OK) rc = balloc(sizeof(SegHost), (void**)&B->seg_d);
   // run products: 2 kBatchSubRecs sieving primes below maxT + 1 take at most
-  // 2 kBatchSubRecs bits(maxT) bits
+  // 2 kBatchSubRecs bits(maxT) bits (sized with one bit more per prime)
   {
-    const uint32_t bits = B->maxT ? 64 - __builtin_clzll(B->maxT) : 1;
+    const uint32_t bits = (B->maxT ? 64 - __builtin_clzll(B->maxT) : 1) + 1;
     const uint32_t digits = (2 * kBatchSubRecs * bits + LB_DIGIT - 1) / LB_DIGIT;
     B->ds = std::min<uint32_t>(kBatchSubMaxDigits, std::max<uint32_t>(12, (digits + 11) / 12 * 12));
   }
@@ -853,7 +855,7 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   h.lo = q;
   h.hi = q;
   h.span = span_blocks * CLW_BLOCK_SPAN;
-  h.end = q + h.span;
+  h.end = std::min(q + h.span, B->maxT + 1);  // run products are sized for primes up to maxT
   CK(clw_h2d(ctx, B->seg_d, &h, sizeof(h), 1));
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
