@@ -933,10 +933,10 @@ extern "C" int clw_device_primes(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_
   uint32_t hflags = 0;
   if (rc == CLW_OK) rc = clw_d2h(ctx, &hflags, flags, 4, 1);
   if (rc == CLW_OK && hflags) rc = clw_fail(CLW_ERR_OVERFLOW, "clw_device_primes: sieve capacity exceeded");
-  hipFree(counts);
-  hipFree(flags);
-  hipFree(offs);
-  hipFree(dout);
+  for (void* x : {(void*)counts, (void*)flags, (void*)offs, (void*)dout}) {
+    const hipError_t fe = x ? hipFree(x) : hipSuccess;
+    if (fe != hipSuccess && rc == CLW_OK) rc = clw_fail_hip(fe, "clw_device_primes: hipFree");
+  }
   *count = n;
   return rc;
 }
