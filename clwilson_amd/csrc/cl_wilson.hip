@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -558,11 +559,42 @@ extern "C" int clw_session_finalize(clw_session* S, clw_range_stats* stats, uint
   return CLW_OK;
 }
 
+// Ranges with at least this many test primes (estimated as width / ln pmin)
+// run through a one-range batch in the automatic mode: the batched engine's
+// run products are ~2x faster per prime than the session's record products,
+// while the session's wave-per-test-prime kernel suits small ranges.
+static const double kBatchMinTestPrimes = 65536;
+
+static int wilson_range_batch(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
+                              clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot, uint64_t* out_rem,
+                              uint64_t out_cap) {
+  clw_options bo = opt ? *opt : clw_options{};
+  bo.mode = 0;
+  clw_batch* B = nullptr;
+  CK(clw_batch_create(ctx, &pmin, &pmax, 1, &bo, &B));
+  const auto t0 = std::chrono::steady_clock::now();
+  clw_range_stats info{};
+  int rc = clw_batch_info(B, 0, &info);
+  if (rc == CLW_OK) rc = clw_batch_run(B, 2, info.maxtarget + 1);
+  if (rc == CLW_OK) rc = clw_batch_iterate(B);
+  if (rc == CLW_OK) rc = clw_batch_finalize(B);
+  if (rc == CLW_OK) rc = clw_batch_results(B, 0, stats, out_p, out_quot, out_rem, out_cap);
+  if (rc == CLW_OK) {
+    stats->totalcount = clw_batch_applied(B);  // one range: pi(its largest target)
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  const int rd = clw_batch_destroy(B);
+  return rc != CLW_OK ? rc : rd;
+}
+
 extern "C" int clw_wilson_range(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
                                 clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot, uint64_t* out_rem,
                                 uint64_t out_cap) {
   if (!ctx || !stats) return clw_fail(CLW_ERR_ARG, "clw_wilson_range: null argument");
   memset(stats, 0, sizeof(*stats));
+  if ((!opt || opt->mode == 0) && pmin >= 5 && pmin < pmax && pmax - pmin <= 10000000 &&
+      clw_limbs_for(pmax - 1) <= 4 && (double)(pmax - pmin) / std::log((double)pmin) >= kBatchMinTestPrimes)
+    return wilson_range_batch(ctx, pmin, pmax, opt, stats, out_p, out_quot, out_rem, out_cap);
   clw_session* S = nullptr;
   CK(clw_session_create(ctx, pmin, pmax, opt, &S));
   int rc = clw_session_run(S, 2, S->maxT + 1);

@@ -216,7 +216,8 @@ typedef struct {
   uint64_t segment_span;   /* integers per sieve segment (0 = default)        */
   uint32_t target_waves;   /* waves to aim for per product launch (0 = auto)  */
   uint32_t verbose;        /* progress lines on stderr                        */
-  uint32_t mode;           /* product kernel: 0 auto, 1 lane per test prime
+  uint32_t mode;           /* product kernel: 0 auto (clw_wilson_range: a
+                              one-range batch for large ranges), 1 lane per test prime
                               (clw_mul_records), 2 wave per test prime
                               (clw_mul_records_lanes), 3 the reference's
                               launch structure, one test prime per launch
@@ -251,7 +252,11 @@ typedef struct {
 
 /* Runs the plain method on [pmin, pmax) (5 <= pmin < pmax <= 2^62-1, width
  * <= 10^7). If out_* are non-NULL they receive, in increasing p, the test
- * primes, their Wilson quotients quot = ((p-1)!+1)/p mod p and rem. */
+ * primes, their Wilson quotients quot = ((p-1)!+1)/p mod p and rem. In the
+ * automatic mode a range with about 65536 test primes or more (width / ln
+ * pmin) and p below 8.2e17 runs as a one-range batch (same results, the
+ * batched engine's faster product path); otherwise, and for modes 1-3, a
+ * session. */
 int clw_wilson_range(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, const clw_options* opt,
                      clw_range_stats* stats, uint64_t* out_p, uint64_t* out_quot, uint64_t* out_rem,
                      uint64_t out_cap);

@@ -48,10 +48,13 @@ def test_small_ranges_bit_exact(dev, golden, mode):
         assert res.totalcount == case["pi_maxt"]
 
 
-def test_config1_full_range(dev, golden):
-    """[1e6, 2e6): 70,435 primes; the quotient vector hashes to the oracle's."""
+@pytest.mark.parametrize("mode", [0, 1])
+def test_config1_full_range(dev, golden, mode):
+    """[1e6, 2e6): 70,435 primes; the quotient vector hashes to the oracle's,
+    through the automatic mode (a one-range batch at this size) and the
+    session's lane-per-test-prime kernel."""
     g = golden("cfg1.json")
-    res = dev.wilson_range(g["pmin"], g["pmax"])
+    res = dev.wilson_range(g["pmin"], g["pmax"], mode=mode)
     assert len(res.p) == g["tpcount"]
     assert hashlib.sha256(np.ascontiguousarray(res.p, dtype="<u8").tobytes()).hexdigest() == g["p_sha256"]
     assert hashlib.sha256(np.ascontiguousarray(res.quot, dtype="<u8").tobytes()).hexdigest() == g["quot_sha256"]
@@ -195,7 +198,7 @@ def test_every_prime_below_2e6_vs_definition(dev, golden, engine):
     [5, 1e6) and [1e6, 2e6), through the session path and the batched engine."""
     for d in golden("direct_lt_2e6.json"):
         if engine == "session":
-            r = dev.wilson_range(d["pmin"], d["pmax"])
+            r = dev.wilson_range(d["pmin"], d["pmax"], mode=1)
         else:
             with cw.Batch(dev, [(d["pmin"], d["pmax"])]) as b:
                 r = b.search()[0]
