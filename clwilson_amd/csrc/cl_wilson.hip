@@ -189,6 +189,11 @@ struct clw_session {
   clw_seg_buffers seg{};
   uint64_t* part_lo = nullptr;
   uint64_t* part_hi = nullptr;
+  // run products of a segment's sub-chunks (lane-per-test-prime mode, 2-4
+  // limbs): digits with stride ds per (block, sub-chunk), their counts
+  uint32_t* subp = nullptr;
+  uint32_t* subn = nullptr;
+  uint32_t ds = 0;
   uint64_t total_primes = 0;  // sieving primes applied so far
   bool iterated = false;
   double gpu_seconds = 0;
@@ -205,6 +210,8 @@ struct clw_session {
     clw_seg_free(ctx, &seg);
     clw_free(ctx, part_lo);
     clw_free(ctx, part_hi);
+    clw_free(ctx, subp);
+    clw_free(ctx, subn);
   }
 };
 
@@ -295,6 +302,16 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
       kPlainGroups);
   if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_lo);
   if (rc == CLW_OK) rc = clw_alloc(ctx, part_entries * 8, (void**)&S->part_hi);
+  if (rc == CLW_OK && !S->lanes_mode && S->opt.mode != 3 && S->tab.limbs <= 4) {
+    // 2 kBatchSubRecs sieving primes up to maxT: at most 2 kBatchSubRecs
+    // (bits(maxT) + 1) bits, in 30-bit digits rounded up to 12
+    const uint32_t bits = (S->maxT ? 64 - __builtin_clzll(S->maxT) : 1) + 1;
+    const uint32_t digits = (2 * kBatchSubRecs * bits + 29) / 30;
+    S->ds = std::min<uint32_t>(kBatchSubMaxDigits, std::max<uint32_t>(12, (digits + 11) / 12 * 12));
+    const size_t slots = (size_t)S->seg.max_blocks * kBatchSubCap;
+    rc = clw_alloc(ctx, slots * S->ds * 4, (void**)&S->subp);
+    if (rc == CLW_OK) rc = clw_alloc(ctx, slots * 4, (void**)&S->subn);
+  }
   if (rc == CLW_OK) rc = clw_finish(ctx);
   if (rc == CLW_OK) {
     S->ev_ok = true;
@@ -375,6 +392,7 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       uint32_t nb = 0;
       CK(clw_hip(hipEventRecord(S->ev[0], st), "hipEventRecord"));
       CK(clw_sieve_segment(ctx, &S->seg, lo, hi, S->T, &nb));
+      if (S->subp) CK(clwi_session_subprod(ctx, S->seg.rec, S->seg.nrec, nb, S->subp, S->subn, S->ds, S->seg.flags));
       CK(clw_hip(hipEventRecord(S->ev[1], st), "hipEventRecord"));
       ran_sieve = true;
     }
@@ -415,8 +433,12 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
         K = std::min<uint32_t>(K, std::min<uint32_t>(kMaxKRec, std::max<uint32_t>(nblk, 1)));
         Ks = std::min<uint32_t>(kMaxKSide, std::max<uint32_t>(1, K / 4));
         if (nblk) {
-          CK(clw_mul_records(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n,
-                             per_lane));
+          if (S->subp)
+            CK(clwi_mul_records_runs(ctx, &S->tab, &S->seg, S->subp, S->subn, S->ds, t, S->off[t], n, nblk, K,
+                                     S->part_lo, S->part_hi, n, per_lane));
+          else
+            CK(clw_mul_records(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n,
+                               per_lane));
           nparts = K;
         }
       }
@@ -460,7 +482,8 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
         for (uint32_t b = 0; b < nblk_t[t]; ++b) recs += nrec_h[b], prs += npr_h[b];
         S->ks.mul_record_apps += recs * S->cnt[t];
         S->ks.mul_prime_apps += prs * S->cnt[t];
-        S->ks.mul_partial_products += recs * S->cnt[t] * (2 * S->tab.limbs * S->tab.limbs + S->tab.limbs);
+        if (!S->subp)  // record products only (with run products the batched engine counts them)
+          S->ks.mul_partial_products += recs * S->cnt[t] * (2 * S->tab.limbs * S->tab.limbs + S->tab.limbs);
       }
       S->ks.side_launches++;
       S->ks.side_ms += ev_ms(e4[1], e4[2]);

@@ -1126,8 +1126,147 @@ extern "C" int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block,
   if (ds % 12 || ds > kBatchSubMaxDigits) return clw_fail(CLW_ERR_ARG, "run products: bad digit stride %u", ds);
   const uint32_t threads = max_blocks * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
-                     (const BatchSeg*)seg, extra_block, rec, nrec, subp, subn, ds, flags);
+                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, flags);
   return clw_check_launch("subchunk_prod");
+}
+
+// Run products of a session segment (nblk sieve blocks of session records).
+extern "C" int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uint32_t* nrec, uint32_t nblk,
+                                    uint32_t* subp, uint32_t* subn, uint32_t ds, uint32_t* flags) {
+  if (ds % 12 || ds > kBatchSubMaxDigits) return clw_fail(CLW_ERR_ARG, "run products: bad digit stride %u", ds);
+  if (nblk == 0) return CLW_OK;
+  const uint32_t threads = nblk * kBatchSubCap;
+  hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
+                     (const BatchSeg*)nullptr, 0u, nblk, rec, nrec, subp, subn, ds, flags);
+  return clw_check_launch("subchunk_prod");
+}
+
+// k_mul_records with run products: a sub-chunk (kSubRecs records) whose
+// first and last records carry the same nonzero exponent of this type has it
+// throughout (along a block a type's record exponents are 0 while they would
+// reach E_SIDE, their side-list range, then non-increasing, then 0 above T),
+// so it is applied as X <- X * REDC_long(P) of the sub-chunk's product P; the
+// other sub-chunks record by record as in k_mul_records.
+template <int L, int KP>
+__global__ __launch_bounds__(256) void k_mul_records_runs(CTX_PARAMS, const uint32_t* __restrict__ rec,
+                                                          const uint32_t* __restrict__ nrec,
+                                                          const uint32_t* __restrict__ subp,
+                                                          const uint32_t* __restrict__ subn, uint32_t ds,
+                                                          uint64_t* __restrict__ part_lo,
+                                                          uint64_t* __restrict__ part_hi, uint32_t tp_off,
+                                                          uint32_t tp_n, uint32_t nblk, uint32_t K,
+                                                          uint32_t part_stride, uint32_t type) {
+  const uint32_t base = blockIdx.x * 256 * KP + threadIdx.x;
+  Ctx<L> c[KP];
+  Res<L> nn[KP];
+  uint32_t ni[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const uint32_t li = base + 256 * j;
+    c[j] = LOAD_CTX(tp_off + (li < tp_n ? li : 0));
+    nn[j] = c[j].n;
+    ni[j] = c[j].ninv;
+  }
+  const uint32_t chunk = blockIdx.y;
+  const uint32_t b0 = (uint32_t)((uint64_t)nblk * chunk / K);
+  const uint32_t b1 = (uint32_t)((uint64_t)nblk * (chunk + 1) / K);
+  Res<L> A[KP], X[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) A[j] = X[j] = res_small<L>(1);
+  uint64_t dA = 0, dX = 0;
+  uint32_t cur = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t nr = nrec[b];
+    const uint4* __restrict__ rp = reinterpret_cast<const uint4*>(rec + (size_t)b * CLW_REC_CAP * CLW_REC_WORDS);
+    const uint32_t* __restrict__ sdig = subp + (size_t)b * kSubCap * ds;
+    const uint32_t* __restrict__ snum = subn + (size_t)b * kSubCap;
+    for (uint32_t s0 = 0; s0 < nr; s0 += kSubRecs) {
+      const uint32_t s1 = min(nr, s0 + kSubRecs);
+      const uint4 f1 = rp[2 * s0 + 1], l1 = rp[2 * (s1 - 1) + 1];
+      const uint32_t ef = type == 0 ? f1.y : (type == 1 ? f1.z : f1.w);
+      const uint32_t el = type == 0 ? l1.y : (type == 1 ? l1.z : l1.w);
+      if (ef != 0 && ef == el) {
+        if (ef != cur) {
+          if (cur) flush_run_k<L, KP>(c, A, dA, X, dX, cur);
+#pragma unroll
+          for (int j = 0; j < KP; ++j) X[j] = res_small<L>(1);
+          dX = 0;
+          cur = ef;
+        }
+        constexpr uint32_t G = redc_span<L>();
+        const uint32_t nd = (snum[s0 / kSubRecs] + G - 1) / G * G;
+        Res<L> Y[KP];
+        redc_long<L, KP>(Y, sdig + (size_t)(s0 / kSubRecs) * ds, nd, nn, ni);
+#pragma unroll
+        for (int j = 0; j < KP; ++j) X[j] = mul_full<L>(X[j], Y[j], nn[j], ni[j]);
+        dX += nd / L + 1;
+        continue;
+      }
+      uint4 w0 = rp[2 * s0], w1 = rp[2 * s0 + 1];
+      for (uint32_t r = s0; r < s1; ++r) {
+        const uint32_t rn = (r + 1 < s1) ? r + 1 : r;
+        const uint4 v0 = rp[2 * rn], v1 = rp[2 * rn + 1];  // next record, in flight during this one
+        const uint32_t e = type == 0 ? w1.y : (type == 1 ? w1.z : w1.w);
+        if (e != 0) {
+          if (e != cur) {
+            if (cur) flush_run_k<L, KP>(c, A, dA, X, dX, cur);
+#pragma unroll
+            for (int j = 0; j < KP; ++j) X[j] = res_small<L>(1);
+            dX = 0;
+            cur = e;
+          }
+          const uint32_t wq[4] = {w0.x, w0.y, w0.z, w0.w};
+          Res<L> q;
+#pragma unroll
+          for (int l = 0; l < L; ++l) q.v[l] = wq[l];
+          Res<L> Xn[KP];
+          montmul_lockstep<L, KP>(Xn, X, q, nn, ni);
+#pragma unroll
+          for (int j = 0; j < KP; ++j) X[j] = Xn[j];
+          ++dX;
+        }
+        w0 = v0;
+        w1 = v1;
+      }
+    }
+  }
+  if (cur) flush_run_k<L, KP>(c, A, dA, X, dX, cur);
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const uint32_t li = base + 256 * j;
+    const Res<L> P = undrift<L>(c[j], A[j], dA);
+    if (li < tp_n) {
+      const u128 v = res_to_u128<L>(P);
+      const size_t o = (size_t)chunk * part_stride + li;
+      part_lo[o] = (uint64_t)v;
+      part_hi[o] = (uint64_t)(v >> 64);
+    }
+  }
+}
+
+// Session products with run products: limb schedules 2-4 (records of 5-limb
+// sessions hold a fifth Q limb: those take clw_mul_records).
+extern "C" int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s,
+                                     const uint32_t* subp, const uint32_t* subn, uint32_t ds, uint32_t type,
+                                     uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K, uint64_t* part_lo,
+                                     uint64_t* part_hi, uint32_t part_stride, uint32_t per_lane) {
+  if (t->limbs > 4 || !subp || !subn)
+    return clw_mul_records(ctx, t, s, type, tp_off, tp_n, nblk, K, part_lo, part_hi, part_stride, per_lane);
+  if (type > 2 || K == 0 || tp_off + tp_n > t->count || nblk > s->max_blocks || tp_n > part_stride)
+    return clw_fail(CLW_ERR_ARG, "clwi_mul_records_runs: bad arguments");
+  if (per_lane == 0) per_lane = clw_default_per_lane(t->limbs);
+  if (tp_n == 0) return CLW_OK;
+  const dim3 grid(ceil_div(tp_n, 256 * per_lane), K), block(256);
+  switch (t->limbs * 8 + per_lane) {
+#define CLW_RUNS_CASE(LL, KK)                                                                                        case LL * 8 + KK:                                                                                                    hipLaunchKernelGGL((k_mul_records_runs<LL, KK>), grid, block, 0, clw_stream(ctx), CTX_ARGS(t), s->rec, s->nrec,                        subp, subn, ds, part_lo, part_hi, tp_off, tp_n, nblk, K, part_stride, type);                    break;
+    CLW_RUNS_CASE(2, 1) CLW_RUNS_CASE(2, 2) CLW_RUNS_CASE(2, 4)
+    CLW_RUNS_CASE(3, 1) CLW_RUNS_CASE(3, 2) CLW_RUNS_CASE(3, 4)
+    CLW_RUNS_CASE(4, 1) CLW_RUNS_CASE(4, 2) CLW_RUNS_CASE(4, 4)
+#undef CLW_RUNS_CASE
+    default:
+      return clw_fail(CLW_ERR_ARG, "unsupported limbs %u / primes per lane %u", t->limbs, per_lane);
+  }
+  return clw_check_launch("mul_records_runs");
 }
 
 extern "C" int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
