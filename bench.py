@@ -83,7 +83,9 @@ class SessionRunner:
         self.T, self.cnt, self.limbs = self.s.type_targets, self.s.tpcnt, self.s.limbs
         self.kp = per_lane or int(cw.lib().clw_default_per_lane(self.limbs))
         lane = mode in (0, 1)
-        self.kernel = f"k_mul_records<{self.limbs},{self.kp}>" if lane else f"k_mul_records_lanes<{self.limbs}>"
+        runs = lane and self.limbs <= 4  # lane mode with run products (2-4 limbs)
+        self.kernel = (f"k_mul_records_runs<{self.limbs},{self.kp}>" if runs else
+                       f"k_mul_records<{self.limbs},{self.kp}>" if lane else f"k_mul_records_lanes<{self.limbs}>")
         self.name = "session, lane per test prime" if lane else "session, wave per test prime"
 
     def run(self, lo, hi):

@@ -194,6 +194,7 @@ struct clw_session {
   uint32_t* subp = nullptr;
   uint32_t* subn = nullptr;
   uint32_t ds = 0;
+  uint64_t* pprod = nullptr;  // device counter: partial products of the run-product kernel
   uint64_t total_primes = 0;  // sieving primes applied so far
   bool iterated = false;
   double gpu_seconds = 0;
@@ -212,6 +213,7 @@ struct clw_session {
     clw_free(ctx, part_hi);
     clw_free(ctx, subp);
     clw_free(ctx, subn);
+    clw_free(ctx, pprod);
   }
 };
 
@@ -311,6 +313,8 @@ extern "C" int clw_session_create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, co
     const size_t slots = (size_t)S->seg.max_blocks * kBatchSubCap;
     rc = clw_alloc(ctx, slots * S->ds * 4, (void**)&S->subp);
     if (rc == CLW_OK) rc = clw_alloc(ctx, slots * 4, (void**)&S->subn);
+    if (rc == CLW_OK) rc = clw_alloc(ctx, 8, (void**)&S->pprod);
+    if (rc == CLW_OK) rc = clw_memset(ctx, S->pprod, 0, 8);
   }
   if (rc == CLW_OK) rc = clw_finish(ctx);
   if (rc == CLW_OK) {
@@ -435,7 +439,7 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
         if (nblk) {
           if (S->subp)
             CK(clwi_mul_records_runs(ctx, &S->tab, &S->seg, S->subp, S->subn, S->ds, t, S->off[t], n, nblk, K,
-                                     S->part_lo, S->part_hi, n, per_lane));
+                                     S->part_lo, S->part_hi, n, per_lane, S->pprod));
           else
             CK(clw_mul_records(ctx, &S->tab, &S->seg, t, S->off[t], n, nblk, K, S->part_lo, S->part_hi, n,
                                per_lane));
@@ -459,6 +463,12 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
       return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded in segment [%llu, %llu): flags %u",
                       (unsigned long long)lo, (unsigned long long)hi, flags[0]);
     S->total_primes += seg_total[0];
+    if (S->pprod) {
+      uint64_t pp = 0;
+      CK(clw_d2h(ctx, &pp, S->pprod, 8, 1));
+      CK(clw_memset(ctx, S->pprod, 0, 8));
+      S->ks.mul_partial_products += pp;
+    }
     // Kernel timings and work of this segment.
     S->ks.segments++;
     if (ran_sieve) {
@@ -482,7 +492,7 @@ extern "C" int clw_session_run(clw_session* S, uint64_t qlo, uint64_t qhi) {
         for (uint32_t b = 0; b < nblk_t[t]; ++b) recs += nrec_h[b], prs += npr_h[b];
         S->ks.mul_record_apps += recs * S->cnt[t];
         S->ks.mul_prime_apps += prs * S->cnt[t];
-        if (!S->subp)  // record products only (with run products the batched engine counts them)
+        if (!S->subp)  // record products only (the run-product kernel counts its own on the device)
           S->ks.mul_partial_products += recs * S->cnt[t] * (2 * S->tab.limbs * S->tab.limbs + S->tab.limbs);
       }
       S->ks.side_launches++;

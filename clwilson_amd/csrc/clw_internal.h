@@ -51,7 +51,7 @@ int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uint32_t* nrec
 int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, const uint32_t* subp,
                           const uint32_t* subn, uint32_t ds, uint32_t type, uint32_t tp_off, uint32_t tp_n,
                           uint32_t nblk, uint32_t K, uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride,
-                          uint32_t per_lane);
+                          uint32_t per_lane, uint64_t* pprod);
 int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
                       const void* plan, const uint64_t* part_lo, const uint64_t* part_hi);
 int clwi_seg_advance(clw_ctx* ctx, void* seg, uint32_t* flags, uint64_t* total_prev, uint64_t* total);

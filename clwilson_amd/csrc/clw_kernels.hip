@@ -1155,8 +1155,10 @@ __global__ __launch_bounds__(256) void k_mul_records_runs(CTX_PARAMS, const uint
                                                           uint64_t* __restrict__ part_lo,
                                                           uint64_t* __restrict__ part_hi, uint32_t tp_off,
                                                           uint32_t tp_n, uint32_t nblk, uint32_t K,
-                                                          uint32_t part_stride, uint32_t type) {
+                                                          uint32_t part_stride, uint32_t type,
+                                                          unsigned long long* __restrict__ pprod) {
   const uint32_t base = blockIdx.x * 256 * KP + threadIdx.x;
+  uint64_t n_full = 0, n_dig = 0;  // full products and digits reduced (uniform)
   Ctx<L> c[KP];
   Res<L> nn[KP];
   uint32_t ni[KP];
@@ -1200,6 +1202,8 @@ __global__ __launch_bounds__(256) void k_mul_records_runs(CTX_PARAMS, const uint
 #pragma unroll
         for (int j = 0; j < KP; ++j) X[j] = mul_full<L>(X[j], Y[j], nn[j], ni[j]);
         dX += nd / L + 1;
+        ++n_full;
+        n_dig += nd;
         continue;
       }
       uint4 w0 = rp[2 * s0], w1 = rp[2 * s0 + 1];
@@ -1224,6 +1228,7 @@ __global__ __launch_bounds__(256) void k_mul_records_runs(CTX_PARAMS, const uint
 #pragma unroll
           for (int j = 0; j < KP; ++j) X[j] = Xn[j];
           ++dX;
+          ++n_full;
         }
         w0 = v0;
         w1 = v1;
@@ -1231,6 +1236,11 @@ __global__ __launch_bounds__(256) void k_mul_records_runs(CTX_PARAMS, const uint
     }
   }
   if (cur) flush_run_k<L, KP>(c, A, dA, X, dX, cur);
+  if (threadIdx.x == 0 && (n_full || n_dig)) {
+    const uint32_t t0 = blockIdx.x * 256 * KP;
+    const uint64_t valid = tp_n - t0 < 256 * KP ? tp_n - t0 : 256 * KP;
+    atomicAdd(pprod, (unsigned long long)((n_full * (2 * L * L + L) + n_dig * (L + 1)) * valid));
+  }
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     const uint32_t li = base + 256 * j;
@@ -1249,20 +1259,28 @@ __global__ __launch_bounds__(256) void k_mul_records_runs(CTX_PARAMS, const uint
 extern "C" int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s,
                                      const uint32_t* subp, const uint32_t* subn, uint32_t ds, uint32_t type,
                                      uint32_t tp_off, uint32_t tp_n, uint32_t nblk, uint32_t K, uint64_t* part_lo,
-                                     uint64_t* part_hi, uint32_t part_stride, uint32_t per_lane) {
-  if (t->limbs > 4 || !subp || !subn)
+                                     uint64_t* part_hi, uint32_t part_stride, uint32_t per_lane, uint64_t* pprod) {
+  if (t->limbs > 4 || !subp || !subn || !pprod)
     return clw_mul_records(ctx, t, s, type, tp_off, tp_n, nblk, K, part_lo, part_hi, part_stride, per_lane);
   if (type > 2 || K == 0 || tp_off + tp_n > t->count || nblk > s->max_blocks || tp_n > part_stride)
     return clw_fail(CLW_ERR_ARG, "clwi_mul_records_runs: bad arguments");
   if (per_lane == 0) per_lane = clw_default_per_lane(t->limbs);
   if (tp_n == 0) return CLW_OK;
   const dim3 grid(ceil_div(tp_n, 256 * per_lane), K), block(256);
+  const auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, clw_stream(ctx), CTX_ARGS(t), s->rec, s->nrec, subp, subn, ds,
+                       part_lo, part_hi, tp_off, tp_n, nblk, K, part_stride, type, (unsigned long long*)pprod);
+  };
   switch (t->limbs * 8 + per_lane) {
-#define CLW_RUNS_CASE(LL, KK)                                                                                        case LL * 8 + KK:                                                                                                    hipLaunchKernelGGL((k_mul_records_runs<LL, KK>), grid, block, 0, clw_stream(ctx), CTX_ARGS(t), s->rec, s->nrec,                        subp, subn, ds, part_lo, part_hi, tp_off, tp_n, nblk, K, part_stride, type);                    break;
-    CLW_RUNS_CASE(2, 1) CLW_RUNS_CASE(2, 2) CLW_RUNS_CASE(2, 4)
-    CLW_RUNS_CASE(3, 1) CLW_RUNS_CASE(3, 2) CLW_RUNS_CASE(3, 4)
-    CLW_RUNS_CASE(4, 1) CLW_RUNS_CASE(4, 2) CLW_RUNS_CASE(4, 4)
-#undef CLW_RUNS_CASE
+    case 2 * 8 + 1: launch(k_mul_records_runs<2, 1>); break;
+    case 2 * 8 + 2: launch(k_mul_records_runs<2, 2>); break;
+    case 2 * 8 + 4: launch(k_mul_records_runs<2, 4>); break;
+    case 3 * 8 + 1: launch(k_mul_records_runs<3, 1>); break;
+    case 3 * 8 + 2: launch(k_mul_records_runs<3, 2>); break;
+    case 3 * 8 + 4: launch(k_mul_records_runs<3, 4>); break;
+    case 4 * 8 + 1: launch(k_mul_records_runs<4, 1>); break;
+    case 4 * 8 + 2: launch(k_mul_records_runs<4, 2>); break;
+    case 4 * 8 + 4: launch(k_mul_records_runs<4, 4>); break;
     default:
       return clw_fail(CLW_ERR_ARG, "unsupported limbs %u / primes per lane %u", t->limbs, per_lane);
   }
