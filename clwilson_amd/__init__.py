@@ -66,7 +66,7 @@ class KernelStats(C.Structure):
                 ("mul_launches", C.c_uint64), ("mul_record_apps", C.c_uint64), ("mul_prime_apps", C.c_uint64),
                 ("side_launches", C.c_uint64), ("reduce_launches", C.c_uint64), ("sieve_ms", C.c_double),
                 ("mul_ms", C.c_double), ("side_ms", C.c_double), ("reduce_ms", C.c_double),
-                ("mul_partial_products", C.c_uint64)]
+                ("mul_partial_products", C.c_uint64), ("mfma_int8_macs", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,7 +97,7 @@ class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
                 ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32),
                 ("segment_apps", C.c_uint64), ("host_primes", C.c_uint32), ("max_chunks", C.c_uint32),
-                ("external_scratch", C.c_uint32)]
+                ("external_scratch", C.c_uint32), ("vector_runs", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -442,7 +442,8 @@ class Batch:
 
     def __init__(self, dev: "Device", ranges: list[tuple[int, int]], segment_span: int = 0,
                  target_waves: int = 0, per_lane: int = 0, graph: bool = True, segment_apps: int = 0,
-                 host_primes: bool = False, max_chunks: int = 0, external_scratch: bool = False):
+                 host_primes: bool = False, max_chunks: int = 0, external_scratch: bool = False,
+                 vector_runs: bool = False):
         if not ranges:
             raise ValueError("a batch needs at least one range")
         self.dev = dev
@@ -451,7 +452,8 @@ class Batch:
         lo = np.array([a for a, _ in self.ranges], dtype=np.uint64)
         hi = np.array([b for _, b in self.ranges], dtype=np.uint64)
         opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0, segment_apps,
-                      1 if host_primes else 0, max_chunks, 1 if external_scratch else 0)
+                      1 if host_primes else 0, max_chunks, 1 if external_scratch else 0,
+                      1 if vector_runs else 0)
         check(lib().clw_batch_create(dev.ctx, lo.ctypes.data, hi.ctypes.data, len(self.ranges), C.byref(opt),
                                      C.byref(self.h)), "batch_create")
         self.info = []

@@ -29,6 +29,9 @@ HIP_FLAGS = [
     "-fPIC",
     # packed FP32 instructions are kept out of every kernel of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in the unified VGPR file: the AGPR form spills and
+    # shuffles the 48 accumulator tiles of k_mfma_runs between iterations
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wno-unused-result",
     f"-I{INCLUDE}",
     f"-I{CSRC}",

@@ -75,6 +75,15 @@ struct Band {
   uint32_t items_per_block = kItemsPerBlock;
   uint64_t active_tp = 0;  // test primes of the groups active in this window
   std::vector<uint64_t> quot, rem;
+  // run products on the matrix cores (3-limb band): tiles of kMfmaTile test
+  // primes of one group, their byte tables and one partial per (slice, test prime)
+  bool mfma = false;
+  std::vector<uint32_t> mt_group, mt_first, mt_n, mt_active;
+  uint32_t* mt_d = nullptr;  // mt_group | mt_first | mt_n | mt_active | active count
+  void* ctab = nullptr;
+  uint64_t* pm_lo = nullptr;
+  uint64_t* pm_hi = nullptr;
+  uint32_t n_mt() const { return (uint32_t)mt_group.size(); }
 };
 
 struct RangeHost {
@@ -136,6 +145,10 @@ struct clw_batch {
   uint32_t* subp = nullptr;    // run products: digits of each (block, sub-chunk)
   uint32_t* subn = nullptr;    // their digit counts
   uint32_t ds = 0;             // digit stride (a multiple of 12)
+  int8_t* subb = nullptr;      // run products as kMfmaKB signed bytes (for a band with mfma)
+  uint32_t* rowlist = nullptr;  // the segment's sub-chunks, dense (k_mfma_rows)
+  uint32_t* mrows = nullptr;
+  bool any_mfma = false;
   SegHost* seg_d = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -166,14 +179,14 @@ struct clw_batch {
     for (Band& b : bands) {
       if (!b.L) continue;
       clw_tp_table_free(ctx, &b.tab);
-      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.plan_d};
+      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.plan_d, b.mt_d, b.ctab, b.pm_lo, b.pm_hi};
       for (void* x : ptrs) clw_free(ctx, x);
       if (b.part_own) {
         clw_free(ctx, b.part_lo);
         clw_free(ctx, b.part_hi);
       }
     }
-    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn};
+    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, rowlist, mrows};
     for (void* x : ptrs) clw_free(ctx, x);
   }
 };
@@ -297,6 +310,13 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   }
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subn);
+  B->any_mfma = B->bands[3].L == 3 && !B->opt.vector_runs;
+  if (B->any_mfma) {
+    B->bands[3].mfma = true;
+    if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * kMfmaKB, (void**)&B->subb);
+    if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->rowlist);
+    if (rc == CLW_OK) rc = balloc(4, (void**)&B->mrows);
+  }
   if (rc == CLW_OK) rc = clw_memset(ctx, B->flags, 0, 16);
   if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, kTotalsBytes);
   for (Band& bd : B->bands) {
@@ -325,6 +345,27 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     if (rc == CLW_OK) rc = balloc(sizeof(PlanHost), (void**)&bd.plan_d);
     if (rc == CLW_OK) rc = balloc((size_t)n * 8, (void**)&bd.from_d);
     if (rc == CLW_OK) rc = clw_h2d(ctx, bd.from_d, bd.from.data(), (size_t)n * 8, 1);
+    if (bd.mfma && rc == CLW_OK) {
+      for (uint32_t gi = 0; gi < bd.groups.size(); ++gi) {
+        const GroupHost& g = bd.groups[gi];
+        for (uint32_t t0 = 0; t0 < g.tp_n; t0 += kMfmaTile) {
+          bd.mt_group.push_back(gi);
+          bd.mt_first.push_back(g.tp_off + t0);
+          bd.mt_n.push_back(std::min(kMfmaTile, g.tp_n - t0));
+        }
+      }
+      const uint32_t nm = bd.n_mt();
+      std::vector<uint32_t> words(4 * (size_t)nm + 1, 0);
+      std::copy(bd.mt_group.begin(), bd.mt_group.end(), words.begin());
+      std::copy(bd.mt_first.begin(), bd.mt_first.end(), words.begin() + nm);
+      std::copy(bd.mt_n.begin(), bd.mt_n.end(), words.begin() + 2 * (size_t)nm);
+      rc = balloc(words.size() * 4, (void**)&bd.mt_d);
+      if (rc == CLW_OK) rc = clw_h2d(ctx, bd.mt_d, words.data(), words.size() * 4, 1);
+      if (rc == CLW_OK) rc = balloc((size_t)nm * kMfmaTileBytes, &bd.ctab);
+      if (rc == CLW_OK) rc = balloc((size_t)kMfmaSlices * n * 8, (void**)&bd.pm_lo);
+      if (rc == CLW_OK) rc = balloc((size_t)kMfmaSlices * n * 8, (void**)&bd.pm_hi);
+      if (rc == CLW_OK) rc = clwi_mfma_table(ctx, &bd.tab, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm, nm, bd.ctab);
+    }
     if (B->opt.external_scratch) {  // chunk products in a caller's buffer (clw_batch_set_scratch)
       bd.part_cap = 0;
       bd.part_own = false;
@@ -450,6 +491,16 @@ static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
     bd.plan.K = K;
     CK(clw_h2d(B->ctx, bd.groups_d, bd.groups.data(), bd.groups.size() * sizeof(GroupHost), 0));
     if (ipc) CK(clw_h2d(B->ctx, bd.item_gt_d, bd.item_gt.data(), (size_t)ipc * 4, 0));
+    if (bd.mfma) {  // the tiles of the active groups, then their count
+      const uint32_t nm = bd.n_mt();
+      bd.mt_active.clear();
+      for (uint32_t mt = 0; mt < nm; ++mt)
+        if (bd.groups[bd.mt_group[mt]].T >= q0) bd.mt_active.push_back(mt);
+      bd.mt_active.push_back((uint32_t)bd.mt_active.size());
+      const size_t na = bd.mt_active.size() - 1;
+      if (na) CK(clw_h2d(B->ctx, bd.mt_d + 3 * (size_t)nm, bd.mt_active.data(), na * 4, 0));
+      CK(clw_h2d(B->ctx, bd.mt_d + 4 * (size_t)nm, &bd.mt_active[na], 4, 0));
+    }
     CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 1));
     act += bd.active_tp;
   }
@@ -478,19 +529,28 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
   CK(clwi_batch_subprod(ctx, B->seg_d, extra_block, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
-                        B->flags));
+                        B->any_mfma ? B->subb : nullptr, B->flags));
+  if (B->any_mfma) CK(clwi_mfma_rows(ctx, B->seg_d, extra_block, B->nrec, B->rowlist, B->mrows));
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
     Band& bd = B->bands[band];
     if (!bd.L) continue;
+    if (bd.mfma) {
+      const uint32_t nm = bd.n_mt();
+      CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
+                        bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->rec,
+                        B->nrec, B->subn, B->subb, bd.ctab, bd.pm_lo, bd.pm_hi, B->totals + 2));
+    }
     CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
                       B->seg_d, (uint32_t)(band - 2), extra_block, B->subp, B->subn, B->ds, bd.resident,
-                      bd.part_lo, bd.part_hi, B->totals + 2));
+                      bd.part_lo, bd.part_hi, B->totals + 2, bd.mfma ? 1u : 0u));
   }
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
     Band& bd = B->bands[band];
-    if (bd.L) CK(clwi_batch_reduce(ctx, &bd.tab, bd.tp_group_d, bd.groups_d, bd.plan_d, bd.part_lo, bd.part_hi));
+    if (bd.L)
+      CK(clwi_batch_reduce(ctx, &bd.tab, bd.tp_group_d, bd.groups_d, bd.plan_d, bd.part_lo, bd.part_hi,
+                           bd.mfma ? bd.pm_lo : nullptr, bd.mfma ? bd.pm_hi : nullptr));
   }
   CK(mark(B, timed));
   return CLW_OK;
@@ -610,7 +670,7 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   clw_ctx* ctx = B->ctx;
   const uint64_t qlo = B->pending.qlo, qhi = B->pending.qhi;
   uint32_t flags[4] = {0, 0, 0, 0};
-  uint64_t tot[5] = {0, 0, 0, 0, 0};
+  uint64_t tot[6] = {0, 0, 0, 0, 0, 0};
   CK(clw_d2h(ctx, flags, B->flags, 16, 0));
   CK(clw_d2h(ctx, tot, B->totals, sizeof(tot), 1));
   if (flags[0]) return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded: flags %u", flags[0]);
@@ -620,6 +680,7 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   B->ks.mul_record_apps += tot[2];
   B->ks.mul_prime_apps += tot[3];
   B->ks.mul_partial_products += tot[4];
+  B->ks.mfma_int8_macs += tot[5];
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   for (uint32_t k = 0; k + 4 <= B->ev_used; k += 4) {
     float a = 0, b = 0, c = 0;
@@ -860,7 +921,8 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
-  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds, B->flags));
+  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds, nullptr,
+                        B->flags));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = clw_hip(hipEventCreate(&e0), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventCreate(&e1), "hipEventCreate");
@@ -868,7 +930,7 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   if (rc == CLW_OK)
     rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
                         (uint32_t)(bd.L - 2), 0, B->subp, B->subn, B->ds, bd.resident, bd.part_lo, bd.part_hi,
-                        B->totals + 2);
+                        B->totals + 2, 0u);
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e1, clw_stream(ctx)), "hipEventRecord");
   uint64_t tot[5] = {0, 0, 0, 0, 0};
   if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, sizeof(tot), 1);

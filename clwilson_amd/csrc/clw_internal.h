@@ -35,6 +35,19 @@ constexpr uint32_t kBatchSubRecs = 64;
 constexpr uint32_t kBatchSubCap = CLW_REC_CAP / kBatchSubRecs;
 constexpr uint32_t kBatchSubMaxDigits = 276;
 
+// Run products on the matrix cores (3-limb band): each sub-chunk product P
+// is also kept as kMfmaKB signed bytes a_i (P = sum a_i 2^(8 i), a_i in
+// [-128, 127]); every test prime of the band has a table of the
+// kMfmaCols signed bytes of c_i = 2^(8 i + 30) mod p^2 for i < kMfmaKB, so that
+// sum_i a_i c_i = P 2^30 (mod p^2) is an int8 GEMM (rows: sub-chunks, columns:
+// test primes x byte columns). kMfmaTile test primes share one table tile in
+// LDS; the rows of a segment are split into kMfmaSlices slices.
+constexpr uint32_t kMfmaTile = 16;
+constexpr uint32_t kMfmaCols = 12;   // bytes of c < p^2 < 2^89
+constexpr uint32_t kMfmaKB = 768;    // bytes per product: 128 primes below 2^44 take <= 705
+constexpr uint32_t kMfmaSlices = 8;
+constexpr size_t kMfmaTileBytes = (size_t)kMfmaCols * kMfmaKB * kMfmaTile;  // 147,456
+
 // Batched-engine launchers (clw_kernels.hip) used by clw_batch.hip.
 extern "C" {
 int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uint32_t max_blocks, uint32_t* flags,
@@ -43,9 +56,19 @@ int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_lane, uint32_
 int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
                    const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec, void* seg,
                    uint32_t qslot, uint32_t extra_block, const uint32_t* subp, const uint32_t* subn, uint32_t ds,
-                   uint32_t grid, uint64_t* part_lo, uint64_t* part_hi, uint64_t* apps);
+                   uint32_t grid, uint64_t* part_lo, uint64_t* part_hi, uint64_t* apps, uint32_t skip_runs);
 int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec, const uint32_t* nrec,
-                       uint32_t max_blocks, uint32_t* subp, uint32_t* subn, uint32_t ds, uint32_t* flags);
+                       uint32_t max_blocks, uint32_t* subp, uint32_t* subn, uint32_t ds, int8_t* subb,
+                       uint32_t* flags);
+int clwi_mfma_table(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* mt_first, const uint32_t* mt_n,
+                    uint32_t n_mt, void* ctab);
+int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* nrec, uint32_t* rowlist,
+                   uint32_t* mrows);
+int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, const uint32_t* mt_group,
+                   const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active, const uint32_t* n_active,
+                   uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows, const uint32_t* rec,
+                   const uint32_t* nrec, const uint32_t* subn, const int8_t* subb, const void* ctab,
+                   uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps);
 int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uint32_t* nrec, uint32_t nblk, uint32_t* subp,
                          uint32_t* subn, uint32_t ds, uint32_t* flags);
 int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, const uint32_t* subp,
@@ -53,7 +76,8 @@ int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buf
                           uint32_t nblk, uint32_t K, uint64_t* part_lo, uint64_t* part_hi, uint32_t part_stride,
                           uint32_t per_lane, uint64_t* pprod);
 int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
-                      const void* plan, const uint64_t* part_lo, const uint64_t* part_hi);
+                      const void* plan, const uint64_t* part_lo, const uint64_t* part_hi, const uint64_t* pm_lo,
+                      const uint64_t* pm_hi);
 int clwi_seg_advance(clw_ctx* ctx, void* seg, uint32_t* flags, uint64_t* total_prev, uint64_t* total);
 int clwi_small_primes_block(clw_ctx* ctx, uint64_t qlo, uint64_t qhi, uint32_t* rec, uint32_t* nrec, uint32_t blk);
 size_t clwi_batch_seg_size();

@@ -832,6 +832,7 @@ __global__ __launch_bounds__(256) void k_finalize(CTX_PARAMS, const uint64_t* __
 }
 
 #include "clw_batch_kernels.inc"
+#include "clw_mfma_kernels.inc"
 
 // ---------------------------------------------------------------------------
 // Launchers (extern "C", see include/clwilson.h)
@@ -1109,24 +1110,24 @@ extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_
                               const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec,
                               void* seg, uint32_t qslot, uint32_t extra_block, const uint32_t* subp,
                               const uint32_t* subn, uint32_t ds, uint32_t grid, uint64_t* part_lo, uint64_t* part_hi,
-                              uint64_t* apps) {
+                              uint64_t* apps, uint32_t skip_runs) {
   if (grid == 0) return CLW_OK;
   DISPATCH_BK(t->limbs, per_lane,
               hipLaunchKernelGGL((k_batch_mul<LL, KK>), dim3(grid), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
                                  (const BatchGroup*)groups, item_gt, (const BatchBand*)plan, rec, nrec,
                                  (BatchSeg*)seg, qslot, extra_block, subp, subn, ds, part_lo, part_hi,
-                                 (unsigned long long*)apps));
+                                 (unsigned long long*)apps, skip_runs));
   return clw_check_launch("batch_mul");
 }
 
 // One thread per (block, sub-chunk) slot of the record buffer.
 extern "C" int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec,
                                   const uint32_t* nrec, uint32_t max_blocks, uint32_t* subp, uint32_t* subn,
-                                  uint32_t ds, uint32_t* flags) {
+                                  uint32_t ds, int8_t* subb, uint32_t* flags) {
   if (ds % 12 || ds > kBatchSubMaxDigits) return clw_fail(CLW_ERR_ARG, "run products: bad digit stride %u", ds);
   const uint32_t threads = max_blocks * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
-                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, flags);
+                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, subb, flags);
   return clw_check_launch("subchunk_prod");
 }
 
@@ -1137,7 +1138,7 @@ extern "C" int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uin
   if (nblk == 0) return CLW_OK;
   const uint32_t threads = nblk * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
-                     (const BatchSeg*)nullptr, 0u, nblk, rec, nrec, subp, subn, ds, flags);
+                     (const BatchSeg*)nullptr, 0u, nblk, rec, nrec, subp, subn, ds, (int8_t*)nullptr, flags);
   return clw_check_launch("subchunk_prod");
 }
 
@@ -1288,13 +1289,48 @@ extern "C" int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const 
 }
 
 extern "C" int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* tp_group, const void* groups,
-                                 const void* plan, const uint64_t* part_lo, const uint64_t* part_hi) {
+                                 const void* plan, const uint64_t* part_lo, const uint64_t* part_hi,
+                                 const uint64_t* pm_lo, const uint64_t* pm_hi) {
   if (t->count == 0) return CLW_OK;
   DISPATCH_L(t->limbs, hipLaunchKernelGGL(k_batch_reduce<LL>, dim3(ceil_div(t->count, 256)), dim3(256), 0,
                                           clw_stream(ctx), CTX_ARGS(t), t->acc_lo, t->acc_hi, tp_group,
                                           (const BatchGroup*)groups, t->count, (const BatchBand*)plan, part_lo,
-                                          part_hi));
+                                          part_hi, pm_lo, pm_hi));
   return clw_check_launch("batch_reduce");
+}
+
+// Byte tables of the matrix-core run products: n_mt tiles of kMfmaTile test
+// primes (tile mt: band entries mt_first[mt] + [0, mt_n[mt])).
+extern "C" int clwi_mfma_table(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* mt_first, const uint32_t* mt_n,
+                               uint32_t n_mt, void* ctab) {
+  if (t->limbs != 3) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3-limb band only");
+  if (n_mt == 0) return CLW_OK;
+  hipLaunchKernelGGL(k_mfma_table, dim3(ceil_div(n_mt * kMfmaTile, 256)), dim3(256), 0, clw_stream(ctx), t->n_lo,
+                     t->n_hi, mt_first, mt_n, n_mt, (uint4*)ctab);
+  return clw_check_launch("mfma_table");
+}
+
+extern "C" int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* nrec, uint32_t* rowlist,
+                              uint32_t* mrows) {
+  hipLaunchKernelGGL(k_mfma_rows, dim3(1), dim3(1024), 0, clw_stream(ctx), (const BatchSeg*)seg, extra_block, nrec,
+                     rowlist, mrows);
+  return clw_check_launch("mfma_rows");
+}
+
+// grid: kMfmaSlices workgroups per tile of the band (n_mt); those beyond the
+// window's *n_active tiles return at once.
+extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, const uint32_t* mt_group,
+                              const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active,
+                              const uint32_t* n_active, uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows,
+                              const uint32_t* rec, const uint32_t* nrec, const uint32_t* subn, const int8_t* subb,
+                              const void* ctab, uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps) {
+  if (t->limbs != 3) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3-limb band only");
+  if (n_mt == 0) return CLW_OK;
+  if ((uint64_t)n_mt * kMfmaSlices > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "too many test-prime tiles");
+  hipLaunchKernelGGL(k_mfma_runs, dim3(n_mt * kMfmaSlices), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
+                     (const BatchGroup*)groups, mt_group, mt_first, mt_n, mt_active, n_active, rowlist, mrows, rec,
+                     nrec, subn, subb, (const uint4*)ctab, t->count, pm_lo, pm_hi, (unsigned long long*)apps);
+  return clw_check_launch("mfma_runs");
 }
 
 extern "C" int clwi_seg_advance(clw_ctx* ctx, void* seg, uint32_t* flags, uint64_t* total_prev, uint64_t* total) {

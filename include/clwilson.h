@@ -236,6 +236,10 @@ typedef struct {
                               (chunk-product slots per test prime; 0 = 256)  */
   uint32_t external_scratch; /* batches: allocate no chunk-product slots; the
                               caller provides them (clw_batch_set_scratch)   */
+  uint32_t vector_runs;    /* batches: reduce the run products of the 3-limb
+                              band on the vector units (k_batch_mul) instead
+                              of as an int8 GEMM on the matrix cores
+                              (k_mfma_runs, the default); same results        */
 } clw_options;
 
 typedef struct {
@@ -296,6 +300,8 @@ typedef struct {
   uint64_t side_launches, reduce_launches;
   double sieve_ms, mul_ms, side_ms, reduce_ms;
   uint64_t mul_partial_products;
+  uint64_t mfma_int8_macs;  /* batches: int8 multiply-adds issued by the
+                               matrix-core run products                       */
 } clw_kernel_stats;
 int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
 int clw_session_reset_stats(clw_session* s);

@@ -145,6 +145,29 @@ def test_batch_run_products_carry_large_q(dev):
     assert ks["mul_partial_products"] < 7 * ks["mul_prime_apps"]  # ~5.9 per prime at this size
 
 
+def test_run_products_matrix_cores_equal_vector_units(dev):
+    """The 3-limb band's run products as an int8 GEMM on the matrix cores
+    (k_mfma_runs, the default) give the residues of the vector-unit reduction
+    (k_batch_mul) bit for bit: windows across exponent-run boundaries (T/2 of
+    1e12's type-2 target 5e11, and type 1's target 2.5e11), T itself and the
+    large-q region of 2e13."""
+    ranges = [(10**12, 10**12 + 20000), (3 * 10**12, 3 * 10**12 + 20000), (2 * 10**13, 2 * 10**13 + 20000)]
+    windows = [(2, 10**6), (2 * 10**11 + 5 * 10**10 - 10**9, 2 * 10**11 + 5 * 10**10 + 10**9),
+               (5 * 10**11 - 10**9, 5 * 10**11 + 10**9), (10**12, 10**12 + 2 * 10**9)]
+    got = {}
+    for vec in (False, True):
+        with cw.Batch(dev, ranges, vector_runs=vec) as b:
+            for qlo, qhi in windows:
+                b.run(qlo, qhi)
+            got[vec] = ([b.residues(r) for r in range(len(ranges))], b.kernel_stats())
+    for r in range(len(ranges)):
+        assert got[False][0][r] == got[True][0][r], f"range {ranges[r]}"
+    say("int8 MACs", got[False][1]["mfma_int8_macs"], "prime apps", got[False][1]["mul_prime_apps"],
+        got[True][1]["mul_prime_apps"])
+    assert got[False][1]["mfma_int8_macs"] > 0 and got[True][1]["mfma_int8_macs"] == 0
+    assert got[False][1]["mul_prime_apps"] == got[True][1]["mul_prime_apps"]
+
+
 def test_batch_rejects_bad_ranges(dev):
     with pytest.raises(cw.ClwError):
         cw.Batch(dev, [(10, 10)])
