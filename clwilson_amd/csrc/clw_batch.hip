@@ -146,6 +146,8 @@ struct clw_batch {
   uint32_t* subn = nullptr;    // their digit counts
   uint32_t ds = 0;             // digit stride (a multiple of 12)
   int8_t* subb = nullptr;      // run products as kMfmaKB signed bytes (for a band with mfma)
+  void* subq = nullptr;         // per sub-chunk: first and last prime (uint4)
+  uint32_t* subm = nullptr;     // per sub-chunk: primes | bytes << 16
   uint32_t* rowlist = nullptr;  // the segment's sub-chunks, dense (k_mfma_rows)
   uint32_t* mrows = nullptr;
   bool any_mfma = false;
@@ -186,7 +188,7 @@ struct clw_batch {
         clw_free(ctx, b.part_hi);
       }
     }
-    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, rowlist, mrows};
+    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, mrows};
     for (void* x : ptrs) clw_free(ctx, x);
   }
 };
@@ -314,6 +316,8 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   if (B->any_mfma) {
     B->bands[3].mfma = true;
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * kMfmaKB, (void**)&B->subb);
+    if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 16, &B->subq);
+    if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subm);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->rowlist);
     if (rc == CLW_OK) rc = balloc(4, (void**)&B->mrows);
   }
@@ -529,7 +533,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
   CK(clwi_batch_subprod(ctx, B->seg_d, extra_block, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
-                        B->any_mfma ? B->subb : nullptr, B->flags));
+                        B->any_mfma ? B->subb : nullptr, B->subq, B->subm, B->flags));
   if (B->any_mfma) CK(clwi_mfma_rows(ctx, B->seg_d, extra_block, B->nrec, B->rowlist, B->mrows));
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
@@ -538,8 +542,8 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
     if (bd.mfma) {
       const uint32_t nm = bd.n_mt();
       CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
-                        bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->rec,
-                        B->nrec, B->subn, B->subb, bd.ctab, bd.pm_lo, bd.pm_hi, B->totals + 2));
+                        bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
+                        B->subm, B->subb, bd.ctab, bd.pm_lo, bd.pm_hi, B->totals + 2));
     }
     CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
                       B->seg_d, (uint32_t)(band - 2), extra_block, B->subp, B->subn, B->ds, bd.resident,
@@ -922,7 +926,7 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
   CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds, nullptr,
-                        B->flags));
+                        nullptr, nullptr, B->flags));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = clw_hip(hipEventCreate(&e0), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventCreate(&e1), "hipEventCreate");

@@ -1123,11 +1123,12 @@ extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_
 // One thread per (block, sub-chunk) slot of the record buffer.
 extern "C" int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec,
                                   const uint32_t* nrec, uint32_t max_blocks, uint32_t* subp, uint32_t* subn,
-                                  uint32_t ds, int8_t* subb, uint32_t* flags) {
+                                  uint32_t ds, int8_t* subb, void* subq, uint32_t* subm, uint32_t* flags) {
   if (ds % 12 || ds > kBatchSubMaxDigits) return clw_fail(CLW_ERR_ARG, "run products: bad digit stride %u", ds);
   const uint32_t threads = max_blocks * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
-                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, subb, flags);
+                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, subb, (uint4*)subq, subm,
+                     flags);
   return clw_check_launch("subchunk_prod");
 }
 
@@ -1138,7 +1139,8 @@ extern "C" int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uin
   if (nblk == 0) return CLW_OK;
   const uint32_t threads = nblk * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
-                     (const BatchSeg*)nullptr, 0u, nblk, rec, nrec, subp, subn, ds, (int8_t*)nullptr, flags);
+                     (const BatchSeg*)nullptr, 0u, nblk, rec, nrec, subp, subn, ds, (int8_t*)nullptr,
+                     (uint4*)nullptr, (uint32_t*)nullptr, flags);
   return clw_check_launch("subchunk_prod");
 }
 
@@ -1322,14 +1324,15 @@ extern "C" int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, con
 extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, const uint32_t* mt_group,
                               const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active,
                               const uint32_t* n_active, uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows,
-                              const uint32_t* rec, const uint32_t* nrec, const uint32_t* subn, const int8_t* subb,
-                              const void* ctab, uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps) {
+                              const void* subq, const uint32_t* subm, const int8_t* subb, const void* ctab,
+                              uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps) {
   if (t->limbs != 3) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3-limb band only");
   if (n_mt == 0) return CLW_OK;
   if ((uint64_t)n_mt * kMfmaSlices > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "too many test-prime tiles");
-  hipLaunchKernelGGL(k_mfma_runs, dim3(n_mt * kMfmaSlices), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
-                     (const BatchGroup*)groups, mt_group, mt_first, mt_n, mt_active, n_active, rowlist, mrows, rec,
-                     nrec, subn, subb, (const uint4*)ctab, t->count, pm_lo, pm_hi, (unsigned long long*)apps);
+  hipLaunchKernelGGL(k_mfma_runs, dim3(n_mt * kMfmaSlices), dim3(kMfmaThreads), 0, clw_stream(ctx), CTX_ARGS(t),
+                     (const BatchGroup*)groups, mt_group, mt_first, mt_n, mt_active, n_active, rowlist, mrows,
+                     (const uint4*)subq, subm, subb, (const uint4*)ctab, t->count, pm_lo, pm_hi,
+                     (unsigned long long*)apps);
   return clw_check_launch("mfma_runs");
 }
 
