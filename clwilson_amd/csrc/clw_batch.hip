@@ -543,7 +543,8 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
       const uint32_t nm = bd.n_mt();
       CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                         bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
-                        B->subm, B->subb, bd.ctab, bd.pm_lo, bd.pm_hi, B->totals + 2));
+                        B->subm, B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo,
+                        bd.pm_hi, B->totals + 2));
     }
     CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
                       B->seg_d, (uint32_t)(band - 2), extra_block, B->subp, B->subn, B->ds, bd.resident,
