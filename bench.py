@@ -23,11 +23,15 @@ the 256-range batched workload (starts log-uniform in [1e9, 1e14] from seed
 1234, widths uniform in [1e5, 1e7]) is measured the same way at the
 quantiles of its factor-application mass.
 
-The roofline line is for the dominant kernel (the record products): the
-algorithmic 30x30-bit partial products per launch (2L^2 + L per record per
-test prime with L limbs) over the mean launch time from HIP events on the
-library's stream, against gfx950's peak v_mad_u64_u32 issue rate (CUs x 64
-lanes x clock).
+The roofline line is for the dominant kernel. At 3 limbs that is
+k_mfma_runs, the run products as an int8 GEMM on the matrix cores: its
+algorithmic int8 multiply-adds per launch (12 byte columns x the bytes of each
+128-prime sub-chunk product, per test prime it is applied to) x 2 over its
+mean launch time (HIP events on the library's stream around it), against
+MI355X's dense int8 peak of 5.0 POP/s. The vector-unit side (k_batch_mul,
+which keeps the records across run boundaries) is reported beside it in
+30x30-bit partial products against the peak v_mad_u64_u32 issue rate (CUs x
+64 lanes x clock).
 """
 from __future__ import annotations
 
@@ -48,14 +52,15 @@ METRIC = "primes tested/s at p≈2e13 (batched vs plain), % of int-MAD roofline"
 PMIN, PMAX = 20_000_000_000_000, 20_000_010_000_000
 SPAN = 4370 * 61440  # integers per step (one default segment)
 
-# HBM traffic of k_batch_mul<3,3> on one SPAN segment just below T0 (two
-# launches of `bench.py --profile-run` under rocprofv3 --pmc FETCH_SIZE, then
-# --pmc WRITE_SIZE; KiB per dispatch). The counters are scaled by what they
-# read for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB
-# for the 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
-# profiles/batch_mul_cfg3_hbm_counters.md.
-BATCH_MUL_TRAFFIC = dict(fetch_kib=1014349.8, write_kib=496138.1, fetch_scale=2.0, write_scale=1.0,
-                         window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
+# HBM traffic of k_mfma_runs on one SPAN segment just below T0 (two launches
+# of `bench.py --profile-run` under rocprofv3 --pmc FETCH_SIZE, then --pmc
+# WRITE_SIZE; KiB per dispatch). The counters are scaled by what they read
+# for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB for the
+# 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
+# profiles/mfma_runs_cfg3_counters.md.
+MFMA_TRAFFIC = dict(fetch_kib=32013797.0, write_kib=4171220.0, fetch_scale=2.0, write_scale=1.0,
+                    window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
+INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 matrix-core peak (ops/s; a multiply-add is 2)
 
 
 def say(*a):
@@ -118,7 +123,9 @@ class BatchRunner:
         self.T, self.cnt, self.limbs = list(st.typeTarget), list(st.tpcnt), st.limbs
         self.tune = self.b.autotune(st.typeTarget[0]) if tune else None
         self.kp = self.tune[0]["per_lane"] if tune else (per_lane or int(cw.lib().clw_batch_default_per_lane(self.limbs)))
-        self.kernel = f"k_batch_mul<{self.limbs},{self.kp}>"
+        self.vec_kernel = f"k_batch_mul<{self.limbs},{self.kp}>"
+        # 3 limbs: the run products go to the matrix cores
+        self.kernel = "k_mfma_runs" if self.limbs == 3 else self.vec_kernel
         self.name = "batched engine (one-range batch, persistent work-queue kernel)"
         self._base = None
 
@@ -332,8 +339,9 @@ def profile_run(cw, args) -> int:
     run.close()
     dev.close()
     print(json.dumps(dict(copy_bytes=n, kernel=run.kernel, launches=ks["mul_launches"],
-                          mul_ms=ks["mul_ms"], record_apps=ks["mul_record_apps"],
-                          prime_apps=ks["mul_prime_apps"], partial_products=ks["mul_partial_products"])))
+                          mul_ms=ks["mul_ms"], mfma_ms=ks["mfma_ms"], record_apps=ks["mul_record_apps"],
+                          prime_apps=ks["mul_prime_apps"], partial_products=ks["mul_partial_products"],
+                          int8_macs=ks["mfma_int8_macs"], int8_issued=ks["mfma_int8_issued"])))
     return 0
 
 
@@ -363,7 +371,8 @@ def main() -> int:
                          "against a full search); the JSON line then describes that range")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: a 256 MiB device copy (counter calibration), then two "
-                         "one-segment windows of the [2e13, 2e13+1e7) batch (two k_batch_mul launches)")
+                         "one-segment windows of the [2e13, 2e13+1e7) batch (two k_mfma_runs and two k_batch_mul "
+                         "launches)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would search its own "
@@ -441,7 +450,49 @@ def main() -> int:
     # algorithmic 30x30-bit partial products counted by the product kernel:
     # 2L^2 + L per full Montgomery product (records, one per run product),
     # L + 1 per 30-bit digit of a run product reduced mod p^2
-    achieved = ks["mul_partial_products"] / (ks["mul_ms"] * 1e-3) if ks["mul_ms"] else 0.0
+    vec_ms = ks["mul_ms"] - ks["mfma_ms"]
+    achieved = ks["mul_partial_products"] / (vec_ms * 1e-3) if vec_ms > 0 else 0.0
+    vec_line = {
+        "bound": "valu-int32-mad",
+        "kernel": getattr(run, "vec_kernel", run.kernel),
+        "achieved": achieved / 1e12,
+        "peak": peak / 1e12,
+        "unit": "Tmad/s",
+        "frac": achieved / peak if peak else None,
+        "per_unit": f"30x30-bit partial products per test prime: {L + 1} per 30-bit digit of a run product "
+                    f"reduced, {prod_per_rec} per full Montgomery product (records across run boundaries, "
+                    "one per run product)",
+        "partial_products": ks["mul_partial_products"],
+        "launches": ks["mul_launches"],
+        "mean_launch_ms": vec_ms / max(1, ks["mul_launches"]),
+    }
+    if ks["mfma_int8_macs"]:
+        mfma_launches = max(1, ks["sieve_launches"])
+        ops = 2.0 * ks["mfma_int8_macs"] / (ks["mfma_ms"] * 1e-3) if ks["mfma_ms"] else 0.0
+        traffic = None
+        if (MFMA_TRAFFIC["fetch_kib"] and args.range is None and not args.tune_single
+                and run.kernel == "k_mfma_runs"):
+            traffic = (MFMA_TRAFFIC["fetch_kib"] * MFMA_TRAFFIC["fetch_scale"]
+                       + MFMA_TRAFFIC["write_kib"] * MFMA_TRAFFIC["write_scale"]) * 1024
+        roofline = {
+            "bound": "mfma",
+            "kernel": "k_mfma_runs",
+            "achieved": ops / 1e12,
+            "peak": INT8_PEAK_OPS / 1e12,
+            "unit": "TOP/s",
+            "frac": ops / INT8_PEAK_OPS,
+            "traffic": traffic,
+            "traffic_of": MFMA_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
+            "per_unit": "int8 multiply-adds (2 ops each) per (128-prime sub-chunk inside an exponent run, test "
+                        "prime): 12 byte columns x the bytes of the sub-chunk's product (~668 at 2e13: ~8,000)",
+            "int8_macs": ks["mfma_int8_macs"],
+            "int8_macs_issued": ks["mfma_int8_issued"],
+            "launches": mfma_launches,
+            "mean_launch_ms": ks["mfma_ms"] / mfma_launches,
+            "vector_units": vec_line,
+        }
+    else:
+        roofline = vec_line
     batch256 = measure_batch256(dev, cw, args.batch_steps, not args.no_tune) if args.batch_steps > 0 else None
     serving = measure_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     out = {
@@ -471,25 +522,8 @@ def main() -> int:
             "per_lane": run.kp,
             "autotune": getattr(run, "tune", None),
         },
-        "roofline": {
-            "bound": "valu-int32-mad",
-            "kernel": run.kernel,
-            "achieved": achieved / 1e12,
-            "peak": peak / 1e12,
-            "unit": "Tmad/s",
-            "frac": achieved / peak if peak else None,
-            "traffic": (BATCH_MUL_TRAFFIC["fetch_kib"] * BATCH_MUL_TRAFFIC["fetch_scale"]
-                        + BATCH_MUL_TRAFFIC["write_kib"] * BATCH_MUL_TRAFFIC["write_scale"]) * 1024
-            if run.kernel == "k_batch_mul<3,3>" and args.range is None and not args.tune_single else None,
-            "traffic_of": BATCH_MUL_TRAFFIC["window"] + " (rocprofv3 counters, copy-calibrated)",
-            "per_unit": f"30x30-bit partial products per test prime: {L + 1} per 30-bit digit of a run product "
-                        f"reduced, {prod_per_rec} per full Montgomery product (records across run boundaries, "
-                        "one per run product)",
-            "partial_products": ks["mul_partial_products"],
-            "launches": ks["mul_launches"],
-            "mean_launch_ms": ks["mul_ms"] / max(1, ks["mul_launches"]),
-        },
-        "kernel_ms": {k: ks[k] for k in ("sieve_ms", "mul_ms", "reduce_ms")},
+        "roofline": roofline,
+        "kernel_ms": {k: ks[k] for k in ("sieve_ms", "mfma_ms", "mul_ms", "reduce_ms")},
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "batch256": batch256,

@@ -66,7 +66,8 @@ class KernelStats(C.Structure):
                 ("mul_launches", C.c_uint64), ("mul_record_apps", C.c_uint64), ("mul_prime_apps", C.c_uint64),
                 ("side_launches", C.c_uint64), ("reduce_launches", C.c_uint64), ("sieve_ms", C.c_double),
                 ("mul_ms", C.c_double), ("side_ms", C.c_double), ("reduce_ms", C.c_double),
-                ("mul_partial_products", C.c_uint64), ("mfma_int8_macs", C.c_uint64)]
+                ("mul_partial_products", C.c_uint64), ("mfma_int8_macs", C.c_uint64),
+                ("mfma_ms", C.c_double), ("mfma_int8_issued", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -50,6 +50,8 @@ const uint32_t kMaxChunks = 256;
 const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 const size_t kTotalsBytes = 64;            // device counters (8 words)
 const uint32_t LB_DIGIT = 30;              // bits per digit of a run product
+const uint32_t kMarks = 5;                 // events per timed segment: start, sieved, matrix-core
+                                           // run products, products, reductions
 
 inline uint32_t prime_type(uint64_t p) { return p % 3 == 1 ? 0 : (p % 12 == 5 ? 1 : 2); }
 inline uint64_t type_divisor(uint32_t t) { return t == 0 ? 6 : (t == 1 ? 4 : 2); }
@@ -141,7 +143,8 @@ struct clw_batch {
   uint32_t* nrec = nullptr;
   uint32_t* flags = nullptr;
   uint64_t* totals = nullptr;  // primes: [0] this segment, [1] accumulated; products: [2] records,
-                               // [3] primes, [4] 30x30-bit partial products
+                               // [3] primes, [4] 30x30-bit partial products, [5] int8 multiply-adds
+                               // of the matrix-core GEMM (algorithmic), [6] (issued)
   uint32_t* subp = nullptr;    // run products: digits of each (block, sub-chunk)
   uint32_t* subn = nullptr;    // their digit counts
   uint32_t ds = 0;             // digit stride (a multiple of 12)
@@ -157,8 +160,9 @@ struct clw_batch {
   bool iterated = false, finalized = false;
   clw_kernel_stats ks{};
   uint64_t applied = 0;  // sieving primes applied (all windows)
-  // Eager mode: 4 events per segment (start, after sieve, after products,
-  // after reductions), read once per window.
+  // Eager mode: kMarks events per segment (start, after sieve, after the
+  // matrix-core run products, after products, after reductions), read once
+  // per window.
   std::vector<hipEvent_t> events;
   hipEvent_t done_ev = nullptr;  // recorded at the end of each launched window
   uint32_t ev_used = 0;
@@ -538,14 +542,17 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
     Band& bd = B->bands[band];
+    if (!bd.L || !bd.mfma) continue;
+    const uint32_t nm = bd.n_mt();
+    CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
+                      bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq, B->subm,
+                      B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo, bd.pm_hi,
+                      B->totals + 2));
+  }
+  CK(mark(B, timed));
+  for (int band = 2; band <= 4; ++band) {
+    Band& bd = B->bands[band];
     if (!bd.L) continue;
-    if (bd.mfma) {
-      const uint32_t nm = bd.n_mt();
-      CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
-                        bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
-                        B->subm, B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo,
-                        bd.pm_hi, B->totals + 2));
-    }
     CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
                       B->seg_d, (uint32_t)(band - 2), extra_block, B->subp, B->subn, B->ds, bd.resident,
                       bd.part_lo, bd.part_hi, B->totals + 2, bd.mfma ? 1u : 0u));
@@ -675,7 +682,7 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   clw_ctx* ctx = B->ctx;
   const uint64_t qlo = B->pending.qlo, qhi = B->pending.qhi;
   uint32_t flags[4] = {0, 0, 0, 0};
-  uint64_t tot[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t tot[7] = {0, 0, 0, 0, 0, 0, 0};
   CK(clw_d2h(ctx, flags, B->flags, 16, 0));
   CK(clw_d2h(ctx, tot, B->totals, sizeof(tot), 1));
   if (flags[0]) return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded: flags %u", flags[0]);
@@ -686,13 +693,16 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   B->ks.mul_prime_apps += tot[3];
   B->ks.mul_partial_products += tot[4];
   B->ks.mfma_int8_macs += tot[5];
+  B->ks.mfma_int8_issued += tot[6];
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
-  for (uint32_t k = 0; k + 4 <= B->ev_used; k += 4) {
-    float a = 0, b = 0, c = 0;
+  for (uint32_t k = 0; k + kMarks <= B->ev_used; k += kMarks) {
+    float a = 0, m = 0, b = 0, c = 0;
     CK(clw_hip(hipEventElapsedTime(&a, B->events[k], B->events[k + 1]), "hipEventElapsedTime"));
-    CK(clw_hip(hipEventElapsedTime(&b, B->events[k + 1], B->events[k + 2]), "hipEventElapsedTime"));
-    CK(clw_hip(hipEventElapsedTime(&c, B->events[k + 2], B->events[k + 3]), "hipEventElapsedTime"));
+    CK(clw_hip(hipEventElapsedTime(&m, B->events[k + 1], B->events[k + 2]), "hipEventElapsedTime"));
+    CK(clw_hip(hipEventElapsedTime(&b, B->events[k + 1], B->events[k + 3]), "hipEventElapsedTime"));
+    CK(clw_hip(hipEventElapsedTime(&c, B->events[k + 3], B->events[k + 4]), "hipEventElapsedTime"));
     B->ks.sieve_ms += a;
+    B->ks.mfma_ms += m;
     B->ks.mul_ms += b;
     B->ks.reduce_ms += c;
     B->ks.sieve_launches++;

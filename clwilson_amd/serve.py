@@ -168,11 +168,22 @@ class Server:
         """~1.1 width / ln p test primes (an upper estimate)."""
         return 1.1 * (req.pmax - req.pmin) / math.log(max(req.pmin, 3)) + 64
 
+    # the 3-limb band (2^60 <= 2 p^2 < 2^90) keeps byte tables for the
+    # matrix-core run products: 147,456 B per tile of 16 test primes of a group
+    # (~9.2 KB per test prime, up to one partial tile per type) and 8 partials
+    MFMA_BAND = (int(2 ** 29.5), int(2 ** 44.5))
+    MFMA_TILE_BYTES = 147456
+
     @classmethod
     def estimate_bytes(cls, req: Request) -> int:
         """Upper estimate of a request's device footprint: its test primes x
-        (table 88 B + group index, start target and plan ~32 B)."""
-        return int(cls.estimate_primes(req) * 120)
+        (table 88 B + group index, start target and plan ~32 B), plus the
+        byte tables in the 3-limb band."""
+        n = cls.estimate_primes(req)
+        e = n * 120
+        if req.pmax > cls.MFMA_BAND[0] and req.pmin < cls.MFMA_BAND[1]:
+            e += (n / 16 + 3) * cls.MFMA_TILE_BYTES + n * 128
+        return int(e)
 
     @staticmethod
     def magnitude_class(req: Request) -> int:
@@ -182,8 +193,9 @@ class Server:
     def admit(self) -> int:
         """One admission round; returns the number of requests admitted."""
         # record buffer (4096 records of 32 B per block) and run products
-        # (64 sub-chunks of at most 276 digits + a count per block)
-        fixed = (self.segment_span // BLOCK_SPAN + 2) * (4096 * 32 + 64 * 277 * 4) + (1 << 20)
+        # (64 sub-chunks of at most 276 digits + a count per block; for the
+        # matrix cores also 768 bytes, 24 B of row data and a row index)
+        fixed = (self.segment_span // BLOCK_SPAN + 2) * (4096 * 32 + 64 * 277 * 4 + 64 * 796) + (1 << 20)
         classes: dict[int, list[Request]] = {}
         est: dict[int, int] = {}
         primes: dict[int, float] = {}

@@ -300,8 +300,13 @@ typedef struct {
   uint64_t side_launches, reduce_launches;
   double sieve_ms, mul_ms, side_ms, reduce_ms;
   uint64_t mul_partial_products;
-  uint64_t mfma_int8_macs;  /* batches: int8 multiply-adds issued by the
-                               matrix-core run products                       */
+  uint64_t mfma_int8_macs;  /* batches: int8 multiply-adds of the matrix-core
+                               run products (k_mfma_runs): 12 per byte of a
+                               sub-chunk product per test prime it is
+                               applied to (the GEMM's algorithmic count)      */
+  double mfma_ms;           /* batches: k_mfma_runs' share of mul_ms          */
+  uint64_t mfma_int8_issued; /* the multiply-adds its MFMA instructions issue
+                               (rows padded to groups of 48, bytes to 64)     */
 } clw_kernel_stats;
 int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
 int clw_session_reset_stats(clw_session* s);
