@@ -36,7 +36,7 @@ struct GroupHost {  // mirror of BatchGroup
 };
 struct PlanHost {  // mirror of BatchBand
   uint32_t items_per_chunk, K;
-  uint32_t pad[2];
+  uint32_t mfma, pad;
 };
 const uint64_t kNoPart = ~0ull;
 
@@ -50,6 +50,8 @@ const uint32_t kMaxChunks = 256;
 const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 const size_t kTotalsBytes = 64;            // device counters (8 words)
 const uint32_t LB_DIGIT = 30;              // bits per digit of a run product
+const uint64_t kMfmaTableCap = 8ull << 30;  // byte tables kept in HBM up to this size
+const uint32_t kMfmaMinRows = 1024;        // sub-chunks per slice and segment for the matrix cores
 const uint32_t kMarks = 5;                 // events per timed segment: start, sieved, matrix-core
                                            // run products, products, reductions
 
@@ -82,7 +84,8 @@ struct Band {
   bool mfma = false;
   std::vector<uint32_t> mt_group, mt_first, mt_n, mt_active;
   uint32_t* mt_d = nullptr;  // mt_group | mt_first | mt_n | mt_active | active count
-  void* ctab = nullptr;
+  void* ctab = nullptr;      // byte tables in HBM (allocated by the first window that uses them)
+  bool ctab_tried = false;
   uint64_t* pm_lo = nullptr;
   uint64_t* pm_hi = nullptr;
   uint32_t n_mt() const { return (uint32_t)mt_group.size(); }
@@ -185,8 +188,9 @@ struct clw_batch {
     for (Band& b : bands) {
       if (!b.L) continue;
       clw_tp_table_free(ctx, &b.tab);
-      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.plan_d, b.mt_d, b.ctab, b.pm_lo, b.pm_hi};
+      void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.plan_d, b.mt_d, b.pm_lo, b.pm_hi};
       for (void* x : ptrs) clw_free(ctx, x);
+      if (b.ctab) (void)hipFree(b.ctab);
       if (b.part_own) {
         clw_free(ctx, b.part_lo);
         clw_free(ctx, b.part_hi);
@@ -316,7 +320,11 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   }
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subn);
-  B->any_mfma = B->bands[3].L == 3 && !B->opt.vector_runs;
+  if (B->opt.run_products > 2) {
+    delete B;
+    return clw_fail(CLW_ERR_ARG, "clw_options.run_products %u: 0, 1 or 2", B->opt.run_products);
+  }
+  B->any_mfma = B->bands[3].L == 3 && B->opt.run_products != 1;
   if (B->any_mfma) {
     B->bands[3].mfma = true;
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * kMfmaKB, (void**)&B->subb);
@@ -369,10 +377,8 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       std::copy(bd.mt_n.begin(), bd.mt_n.end(), words.begin() + 2 * (size_t)nm);
       rc = balloc(words.size() * 4, (void**)&bd.mt_d);
       if (rc == CLW_OK) rc = clw_h2d(ctx, bd.mt_d, words.data(), words.size() * 4, 1);
-      if (rc == CLW_OK) rc = balloc((size_t)nm * kMfmaTileBytes, &bd.ctab);
       if (rc == CLW_OK) rc = balloc((size_t)kMfmaSlices * n * 8, (void**)&bd.pm_lo);
       if (rc == CLW_OK) rc = balloc((size_t)kMfmaSlices * n * 8, (void**)&bd.pm_hi);
-      if (rc == CLW_OK) rc = clwi_mfma_table(ctx, &bd.tab, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm, nm, bd.ctab);
     }
     if (B->opt.external_scratch) {  // chunk products in a caller's buffer (clw_batch_set_scratch)
       bd.part_cap = 0;
@@ -497,6 +503,7 @@ static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
     }
     bd.plan.items_per_chunk = ipc;
     bd.plan.K = K;
+    bd.plan.mfma = 0;  // set by mfma_window once the segment span is known
     CK(clw_h2d(B->ctx, bd.groups_d, bd.groups.data(), bd.groups.size() * sizeof(GroupHost), 0));
     if (ipc) CK(clw_h2d(B->ctx, bd.item_gt_d, bd.item_gt.data(), (size_t)ipc * 4, 0));
     if (bd.mfma) {  // the tiles of the active groups, then their count
@@ -513,6 +520,44 @@ static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
     act += bd.active_tp;
   }
   *active_total = act;
+  return CLW_OK;
+}
+
+// Run products of the window on the matrix cores or on the vector units: a
+// workgroup of k_mfma_runs loads (or builds) a 147 KB table for its 16 test
+// primes, which pays off over about a thousand sub-chunks of its slice; short
+// segments (large batches at the default segment_apps) keep k_batch_mul's
+// reduction. Sets the plan flag both kernels read and the tile count.
+static int mfma_window(clw_batch* B, uint64_t seg_len, uint64_t qhi) {
+  for (Band& bd : B->bands) {
+    if (!bd.L || !bd.mfma) continue;
+    const double rows = (double)seg_len / std::log((double)std::max<uint64_t>(qhi, 16)) / (2.0 * kBatchSubRecs);
+    const bool use = rows / kMfmaSlices >= kMfmaMinRows && bd.mt_active.size() > 1;
+    bd.plan.mfma = use ? 1u : 0u;
+    const uint32_t nm = bd.n_mt(), na = use ? (uint32_t)bd.mt_active.size() - 1 : 0u;
+    if (use && !bd.ctab_tried) {
+      // the byte tables in HBM (9.2 KB per test prime) up to kMfmaTableCap,
+      // on the first window that uses them; without them (run_products 2,
+      // larger, or no memory) k_mfma_runs builds each tile's table in LDS
+      bd.ctab_tried = true;
+      if (B->opt.run_products == 0 && (uint64_t)nm * kMfmaTileBytes <= kMfmaTableCap &&
+          hipMalloc(&bd.ctab, (size_t)nm * kMfmaTileBytes) == hipSuccess) {
+        B->bytes += (uint64_t)nm * kMfmaTileBytes;
+        CK(clwi_mfma_table(B->ctx, &bd.tab, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm, nm, bd.ctab));
+        if (B->graph_exec) {  // captured without the tables: capture the segment step again
+          CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
+          CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
+          B->graph_exec = nullptr;
+          B->graph = nullptr;
+        }
+      } else {
+        bd.ctab = nullptr;
+        (void)hipGetLastError();  // a failed allocation is not an error here
+      }
+    }
+    CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 0));
+    CK(clw_h2d(B->ctx, bd.mt_d + 4 * (size_t)nm, &na, 4, 1));
+  }
   return CLW_OK;
 }
 
@@ -555,7 +600,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
     if (!bd.L) continue;
     CK(clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec,
                       B->seg_d, (uint32_t)(band - 2), extra_block, B->subp, B->subn, B->ds, bd.resident,
-                      bd.part_lo, bd.part_hi, B->totals + 2, bd.mfma ? 1u : 0u));
+                      bd.part_lo, bd.part_hi, B->totals + 2));
   }
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
@@ -595,6 +640,7 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
     span_blocks = std::min<uint64_t>(span_blocks, std::max<uint64_t>(kMinSpanBlocks, (uint64_t)(apps / per_block) + 1));
   }
   const uint64_t span = span_blocks * CLW_BLOCK_SPAN;
+  CK(mfma_window(B, std::min<uint64_t>(span, qhi - lo), qhi));
   // Host replica of k_seg_advance: the segments of [lo, qhi).
   std::vector<std::pair<uint64_t, uint64_t>> segs;
   for (uint64_t a = lo; a < qhi;) {
@@ -945,7 +991,7 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   if (rc == CLW_OK)
     rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
                         (uint32_t)(bd.L - 2), 0, B->subp, B->subn, B->ds, bd.resident, bd.part_lo, bd.part_hi,
-                        B->totals + 2, 0u);
+                        B->totals + 2);
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e1, clw_stream(ctx)), "hipEventRecord");
   uint64_t tot[5] = {0, 0, 0, 0, 0};
   if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, sizeof(tot), 1);

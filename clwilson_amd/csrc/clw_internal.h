@@ -56,7 +56,7 @@ int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_lane, uint32_
 int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
                    const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec, void* seg,
                    uint32_t qslot, uint32_t extra_block, const uint32_t* subp, const uint32_t* subn, uint32_t ds,
-                   uint32_t grid, uint64_t* part_lo, uint64_t* part_hi, uint64_t* apps, uint32_t skip_runs);
+                   uint32_t grid, uint64_t* part_lo, uint64_t* part_hi, uint64_t* apps);
 int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec, const uint32_t* nrec,
                        uint32_t max_blocks, uint32_t* subp, uint32_t* subn, uint32_t ds, int8_t* subb, void* subq,
                        uint32_t* subm, uint32_t* flags);

@@ -1110,13 +1110,13 @@ extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_
                               const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec,
                               void* seg, uint32_t qslot, uint32_t extra_block, const uint32_t* subp,
                               const uint32_t* subn, uint32_t ds, uint32_t grid, uint64_t* part_lo, uint64_t* part_hi,
-                              uint64_t* apps, uint32_t skip_runs) {
+                              uint64_t* apps) {
   if (grid == 0) return CLW_OK;
   DISPATCH_BK(t->limbs, per_lane,
               hipLaunchKernelGGL((k_batch_mul<LL, KK>), dim3(grid), dim3(256), 0, clw_stream(ctx), CTX_ARGS(t),
                                  (const BatchGroup*)groups, item_gt, (const BatchBand*)plan, rec, nrec,
                                  (BatchSeg*)seg, qslot, extra_block, subp, subn, ds, part_lo, part_hi,
-                                 (unsigned long long*)apps, skip_runs));
+                                 (unsigned long long*)apps));
   return clw_check_launch("batch_mul");
 }
 
@@ -1319,8 +1319,9 @@ extern "C" int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, con
   return clw_check_launch("mfma_rows");
 }
 
-// grid: kMfmaSlices workgroups per tile of the band (n_mt); those beyond the
-// window's *n_active tiles return at once.
+// grid: kMfmaSlices workgroups per tile of the band (n_mt), in rows of at most
+// kMfmaMaxItemsPerLaunch tiles; those beyond the window's *n_active tiles
+// return at once.
 extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, const uint32_t* mt_group,
                               const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active,
                               const uint32_t* n_active, uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows,
@@ -1330,7 +1331,9 @@ extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* g
   if (n_mt == 0) return CLW_OK;
   if ((uint64_t)n_mt * kMfmaSlices > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "too many test-prime tiles");
   if (subb_bytes > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "run-product bytes beyond 2 GiB");
-  hipLaunchKernelGGL(k_mfma_runs, dim3(n_mt * kMfmaSlices), dim3(kMfmaThreads), 0, clw_stream(ctx), CTX_ARGS(t),
+  const dim3 grid(std::min<uint32_t>(n_mt, kMfmaMaxItemsPerLaunch) * kMfmaSlices,
+                  ceil_div(n_mt, kMfmaMaxItemsPerLaunch));
+  hipLaunchKernelGGL(k_mfma_runs, grid, dim3(kMfmaThreads), 0, clw_stream(ctx), CTX_ARGS(t),
                      (const BatchGroup*)groups, mt_group, mt_first, mt_n, mt_active, n_active, rowlist, mrows,
                      (const uint4*)subq, subm, subb, (uint32_t)subb_bytes, (const uint4*)ctab, t->count, pm_lo, pm_hi,
                      (unsigned long long*)apps);

@@ -168,21 +168,20 @@ class Server:
         """~1.1 width / ln p test primes (an upper estimate)."""
         return 1.1 * (req.pmax - req.pmin) / math.log(max(req.pmin, 3)) + 64
 
-    # the 3-limb band (2^60 <= 2 p^2 < 2^90) keeps byte tables for the
-    # matrix-core run products: 147,456 B per tile of 16 test primes of a group
-    # (~9.2 KB per test prime, up to one partial tile per type) and 8 partials
+    # the 3-limb band (2^60 <= 2 p^2 < 2^90) keeps 8 partial products per
+    # test prime for the matrix-core run products (cohorts build their byte
+    # tables per workgroup: run_products=2, no table memory)
     MFMA_BAND = (int(2 ** 29.5), int(2 ** 44.5))
-    MFMA_TILE_BYTES = 147456
 
     @classmethod
     def estimate_bytes(cls, req: Request) -> int:
         """Upper estimate of a request's device footprint: its test primes x
-        (table 88 B + group index, start target and plan ~32 B), plus the
-        byte tables in the 3-limb band."""
+        (table 88 B + group index, start target and plan ~32 B, + 128 B of
+        partial products in the 3-limb band)."""
         n = cls.estimate_primes(req)
         e = n * 120
         if req.pmax > cls.MFMA_BAND[0] and req.pmin < cls.MFMA_BAND[1]:
-            e += (n / 16 + 3) * cls.MFMA_TILE_BYTES + n * 128
+            e += n * 128
         return int(e)
 
     @staticmethod
@@ -231,7 +230,8 @@ class Server:
     def _build(self, take: list[Request]) -> _Cohort:
         """A cohort of requests: their batch built on the admission stream."""
         b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
-                  segment_apps=int(self.rate * self.step_seconds), host_primes=True, external_scratch=True)
+                  segment_apps=int(self.rate * self.step_seconds), host_primes=True, external_scratch=True,
+                  run_products=2)
         groups = [(st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t]]
         c = _Cohort(b, -1, take, [st.maxtarget for st in b.info], np.array([g[0] for g in groups], dtype=np.float64),
                     np.array([g[1] for g in groups], dtype=np.int64), self.now())

@@ -236,10 +236,14 @@ typedef struct {
                               (chunk-product slots per test prime; 0 = 256)  */
   uint32_t external_scratch; /* batches: allocate no chunk-product slots; the
                               caller provides them (clw_batch_set_scratch)   */
-  uint32_t vector_runs;    /* batches: reduce the run products of the 3-limb
-                              band on the vector units (k_batch_mul) instead
-                              of as an int8 GEMM on the matrix cores
-                              (k_mfma_runs, the default); same results        */
+  uint32_t run_products;   /* batches, 3-limb band: 0 = reduce the run
+                              products as an int8 GEMM on the matrix cores
+                              (k_mfma_runs; the byte tables, 9.2 KB per test
+                              prime, kept in HBM up to 8 GiB, else built per
+                              workgroup), 1 = on the vector units
+                              (k_batch_mul), 2 = matrix cores with the tables
+                              always built per workgroup (no table memory);
+                              the same results                                */
 } clw_options;
 
 typedef struct {

@@ -147,25 +147,27 @@ def test_batch_run_products_carry_large_q(dev):
 
 def test_run_products_matrix_cores_equal_vector_units(dev):
     """The 3-limb band's run products as an int8 GEMM on the matrix cores
-    (k_mfma_runs, the default) give the residues of the vector-unit reduction
-    (k_batch_mul) bit for bit: windows across exponent-run boundaries (T/2 of
-    1e12's type-2 target 5e11, and type 1's target 2.5e11), T itself and the
-    large-q region of 2e13."""
+    (k_mfma_runs, the default, with the byte tables from HBM; and mode 2, the
+    tables built by each workgroup in LDS) give the residues of the
+    vector-unit reduction (k_batch_mul, mode 1) bit for bit: windows across
+    exponent-run boundaries (T/2 of 1e12's type-2 target 5e11, and type 1's
+    target 2.5e11), T itself and the large-q region of 2e13."""
     ranges = [(10**12, 10**12 + 20000), (3 * 10**12, 3 * 10**12 + 20000), (2 * 10**13, 2 * 10**13 + 20000)]
     windows = [(2, 10**6), (2 * 10**11 + 5 * 10**10 - 10**9, 2 * 10**11 + 5 * 10**10 + 10**9),
                (5 * 10**11 - 10**9, 5 * 10**11 + 10**9), (10**12, 10**12 + 2 * 10**9)]
     got = {}
-    for vec in (False, True):
-        with cw.Batch(dev, ranges, vector_runs=vec) as b:
+    for mode in (0, 1, 2):
+        with cw.Batch(dev, ranges, run_products=mode) as b:
             for qlo, qhi in windows:
                 b.run(qlo, qhi)
-            got[vec] = ([b.residues(r) for r in range(len(ranges))], b.kernel_stats())
-    for r in range(len(ranges)):
-        assert got[False][0][r] == got[True][0][r], f"range {ranges[r]}"
-    say("int8 MACs", got[False][1]["mfma_int8_macs"], "prime apps", got[False][1]["mul_prime_apps"],
-        got[True][1]["mul_prime_apps"])
-    assert got[False][1]["mfma_int8_macs"] > 0 and got[True][1]["mfma_int8_macs"] == 0
-    assert got[False][1]["mul_prime_apps"] == got[True][1]["mul_prime_apps"]
+            got[mode] = ([b.residues(r) for r in range(len(ranges))], b.kernel_stats())
+    for mode in (0, 2):
+        for r in range(len(ranges)):
+            assert got[mode][0][r] == got[1][0][r], f"mode {mode} range {ranges[r]}"
+        say("mode", mode, "int8 MACs", got[mode][1]["mfma_int8_macs"], "prime apps", got[mode][1]["mul_prime_apps"])
+        assert got[mode][1]["mfma_int8_macs"] > 0
+        assert got[mode][1]["mul_prime_apps"] == got[1][1]["mul_prime_apps"]
+    assert got[1][1]["mfma_int8_macs"] == 0
 
 
 def test_batch_rejects_bad_ranges(dev):
