@@ -86,6 +86,7 @@ struct Band {
   uint32_t* mt_d = nullptr;  // mt_group | mt_first | mt_n | mt_active | active count
   void* ctab = nullptr;      // byte tables in HBM (allocated by the first window that uses them)
   bool ctab_tried = false;
+  bool mfma_lazy = false;   // largest N below 0.4998 R: k_mfma_runs<true>
   uint64_t* pm_lo = nullptr;
   uint64_t* pm_hi = nullptr;
   uint32_t n_mt() const { return (uint32_t)mt_group.size(); }
@@ -327,6 +328,10 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   B->any_mfma = B->bands[3].L == 3 && B->opt.run_products != 1;
   if (B->any_mfma) {
     B->bands[3].mfma = true;
+    // 2 p^2 < 0.9996 R (R = 2^90) for every test prime of the band
+    uint64_t pmax3 = 0;
+    for (uint64_t p : B->bands[3].p) pmax3 = std::max(pmax3, p);
+    B->bands[3].mfma_lazy = 2.0 * (double)pmax3 * (double)pmax3 < 0.9996 * std::ldexp(1.0, 90);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * kMfmaKB, (void**)&B->subb);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 16, &B->subq);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subm);
@@ -592,7 +597,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
     CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                       bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq, B->subm,
                       B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo, bd.pm_hi,
-                      B->totals + 2));
+                      B->totals + 2, bd.mfma_lazy ? 1u : 0u));
   }
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {

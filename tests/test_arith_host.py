@@ -76,6 +76,27 @@ def test_montmul_lazy_bounds(hc, L):
         n_ok += 1
 
 
+def test_montmul_chain_without_cond_sub(hc):
+    """The matrix-core run accumulator (3 limbs) multiplies X by the reduced
+    run product Y < N (1 + 2^-12) with montmul alone: for N < 0.4998 R the
+    chain stays below 2N (and right mod N), from the largest X and Y up."""
+    L, R = 3, 2**90
+    rng = random.Random(11)
+    for p in (sympy.prevprime(int((0.4998 * R) ** 0.5)), sympy.nextprime(2**44), sympy.nextprime(10**13),
+              sympy.nextprime(2**30)):
+        N = p * p
+        assert N < 0.4998 * R
+        ymax = N + (N >> 12)
+        x = 2 * N - 1
+        for k in range(3000):
+            y = ymax - 1 if k % 3 == 0 else rng.randrange(ymax)
+            out = (C.c_uint64 * 2)()
+            hc.clwh_montmul(L, _arr(x), _arr(y), _arr(N), out)
+            r = out[0] | (out[1] << 64)
+            assert r < 2 * N and r % N == x * y * pow(R, -1, N) % N
+            x = r if k % 5 else 2 * N - 1
+
+
 def _column_bound(L, U):
     """Largest 64-bit column sum of montmul_lockstep: limbs of a, b, n below
     2^30, quotient digits m_k below 2^32 for k < U (unmasked), else 2^30."""
