@@ -60,6 +60,7 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # profiles/mfma_runs_cfg3_counters.md.
 MFMA_TRAFFIC = dict(fetch_kib=32013797.0, write_kib=4171220.0, fetch_scale=2.0, write_scale=1.0,
                     window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
+SEG_APPS_256 = 10**14  # factor applications per segment of the 256-range batch
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 matrix-core peak (ops/s; a multiply-add is 2)
 
 
@@ -218,14 +219,17 @@ def measure_plain(dev, cw, regs, nxt, steps):
                 structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
 
 
-def measure_batch256(dev, cw, steps: int, tune: bool, target_apps: float = 4e12) -> dict:
+def measure_batch256(dev, cw, steps: int, tune: bool, target_apps: float = 5e13) -> dict:
     """The 256-range batch: one batch (graph-replayed segment steps), windows
     of the sieving-prime axis at the midpoints of `steps` equal quantiles of
     the factor-application mass F(q) = sum over groups of tp_n x pi(min(q, T));
     each bin's time = its mass / the rate measured in its window."""
     ranges = batch_ranges()
     t0 = time.perf_counter()
-    b = cw.Batch(dev, ranges, graph=True)
+    # segments of ~1e14 factor applications (~2 M sieving primes for the 48 M
+    # test primes): long enough that the 3-limb band's run products go to the
+    # matrix cores (a workgroup's table pays off over ~1,000 sub-chunks)
+    b = cw.Batch(dev, ranges, graph=True, segment_apps=SEG_APPS_256)
     setup = time.perf_counter() - t0
     groups = sorted((st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t])
     ntp = sum(st.tpcount for st in b.info)
