@@ -181,6 +181,7 @@ def lib():
         "clw_batch_run": (i32, [vp, u64, u64]),
         "clw_batch_applied": (u64, [vp]),
         "clw_batch_residues": (i32, [vp, u32, vp, vp, u64]),
+        "clw_batch_set_residues": (i32, [vp, u32, vp, vp, u64, u64, u64]),
         "clw_batch_iterate": (i32, [vp]),
         "clw_batch_finalize": (i32, [vp]),
         "clw_batch_results": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
@@ -537,6 +538,14 @@ class Batch:
         hi = np.zeros(max(n, 1), dtype=np.uint64)
         check(lib().clw_batch_residues(self.h, r, lo.ctypes.data, hi.ctypes.data, n))
         return [int(a) | (int(b) << 64) for a, b in zip(lo[:n], hi[:n])]
+
+    def set_residues(self, r: int, values: list[int], covered: int, applied: int) -> None:
+        """Resume a one-range batch from residues that hold the sieving
+        primes [2, covered) (a checkpoint of the command line's format)."""
+        lo = np.array([v & (2**64 - 1) for v in values], dtype=np.uint64)
+        hi = np.array([v >> 64 for v in values], dtype=np.uint64)
+        check(lib().clw_batch_set_residues(self.h, r, lo.ctypes.data, hi.ctypes.data, len(values), covered,
+                                           applied), "batch_set_residues")
 
     def iterate(self) -> None:
         check(lib().clw_batch_iterate(self.h), "batch_iterate")

@@ -899,6 +899,40 @@ extern "C" int clw_batch_results(const clw_batch* B, uint32_t r, clw_range_stats
   return range_results(B, r, st, out_p, out_quot, out_rem, cap);
 }
 
+// Resume with a caller's own checkpoint format (the command line keeps the
+// reference's, cl_wilson.cpp:1736-1777): range r's canonical residues in
+// increasing p (as clw_batch_residues returns them), holding the sieving
+// primes [2, covered), `applied` of them. One-range batches, before any
+// window of this batch.
+extern "C" int clw_batch_set_residues(clw_batch* B, uint32_t r, const uint64_t* lo, const uint64_t* hi, uint64_t n,
+                                      uint64_t covered, uint64_t applied) {
+  if (!B || !lo || !hi || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_set_residues: bad argument");
+  if (B->ranges.size() != 1) return clw_fail(CLW_ERR_ARG, "clw_batch_set_residues: one-range batches only");
+  if (B->pending.active || B->iterated || B->next_q != 2 || B->applied != 0)
+    return clw_fail(CLW_ERR_STATE, "clw_batch_set_residues: the batch has run windows already");
+  const RangeHost& R = B->ranges[r];
+  if (n != R.tp.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_set_residues: %llu residues for %zu test primes",
+                                        (unsigned long long)n, R.tp.size());
+  for (uint64_t k = 0; k < n; ++k) {
+    const unsigned __int128 v = ((unsigned __int128)hi[k] << 64) | lo[k];
+    if (v >= (unsigned __int128)R.tp[k] * R.tp[k]) return clw_fail(CLW_ERR_ARG, "residue %llu not below p^2", (unsigned long long)k);
+  }
+  Band& bd = B->bands[R.band];
+  const uint32_t cnt = bd.tab.count;
+  std::vector<uint64_t> a(cnt), b(cnt);
+  CK(clw_d2h(B->ctx, a.data(), bd.tab.acc_lo, (size_t)cnt * 8, 0));
+  CK(clw_d2h(B->ctx, b.data(), bd.tab.acc_hi, (size_t)cnt * 8, 1));
+  for (uint64_t k = 0; k < n; ++k) {
+    a[R.pos[k]] = lo[k];
+    b[R.pos[k]] = hi[k];
+  }
+  CK(clw_h2d(B->ctx, bd.tab.acc_lo, a.data(), (size_t)cnt * 8, 0));
+  CK(clw_h2d(B->ctx, bd.tab.acc_hi, b.data(), (size_t)cnt * 8, 1));
+  B->next_q = std::max<uint64_t>(covered, 2);
+  B->applied = applied;
+  return CLW_OK;
+}
+
 extern "C" int clw_batch_residues(clw_batch* B, uint32_t r, uint64_t* lo, uint64_t* hi, uint64_t cap) {
   if (!B || !lo || !hi || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_residues: bad argument");
   const RangeHost& R = B->ranges[r];

@@ -44,6 +44,54 @@ struct Args {
   int device = 0;
 };
 
+// The range's engine: a one-range batch (the batched engine's kernels,
+// matrix-core run products in the 3-limb band) for ranges of at least
+// kBatchMinTestPrimes test primes below 8.2e17, as clw_wilson_range chooses;
+// else a session (a batch builds every sub-chunk product of a segment, which
+// only pays off over many test primes; 5 limbs). Both keep one canonical
+// residue per test prime in increasing p, so the reference's checkpoint
+// files serve either.
+static const uint64_t kBatchMinTestPrimes = 65536;
+struct Engine {
+  clw_session* S = nullptr;
+  clw_batch* B = nullptr;
+  int create(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, uint64_t ntp, const clw_options* opt) {
+    if (ntp >= kBatchMinTestPrimes && clw_limbs_for(pmax - 1) <= 4)
+      return clw_batch_create(ctx, &pmin, &pmax, 1, opt, &B);
+    return clw_session_create(ctx, pmin, pmax, opt, &S);
+  }
+  int info(clw_range_stats* st) { return B ? clw_batch_info(B, 0, st) : clw_session_info(S, st); }
+  int run(uint64_t qlo, uint64_t qhi) { return B ? clw_batch_run(B, qlo, qhi) : clw_session_run(S, qlo, qhi); }
+  int residues(uint64_t* lo, uint64_t* hi, uint64_t n) {
+    return B ? clw_batch_residues(B, 0, lo, hi, n) : clw_session_residues(S, lo, hi, n);
+  }
+  uint64_t applied() {
+    if (B) return clw_batch_applied(B);
+    clw_range_stats cur;
+    clw_session_info(S, &cur);
+    return cur.totalcount;
+  }
+  int resume(const uint64_t* lo, const uint64_t* hi, uint64_t n, uint64_t currp, uint64_t applied) {
+    return B ? clw_batch_set_residues(B, 0, lo, hi, n, currp, applied) : clw_session_set_residues(S, lo, hi, applied);
+  }
+  int finish(clw_range_stats* st, uint64_t* p, uint64_t* q, uint64_t* r, uint64_t n) {
+    if (!B) {
+      int rc = clw_session_iterate(S);
+      if (rc == CLW_OK) rc = clw_session_finalize(S, st, p, q, r, n);
+      return rc;
+    }
+    int rc = clw_batch_iterate(B);
+    if (rc == CLW_OK) rc = clw_batch_finalize(B);
+    if (rc == CLW_OK) rc = clw_batch_results(B, 0, st, p, q, r, n);
+    if (rc == CLW_OK) st->totalcount = clw_batch_applied(B);  // one range: every prime applied to it
+    return rc;
+  }
+  ~Engine() {
+    if (B) clw_batch_destroy(B);
+    if (S) clw_session_destroy(S);
+  }
+};
+
 static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quiet) {
   // setupSearch (cl_wilson.cpp:1146-1182)
   if (a.pmin == 0 || a.pmax == 0) {
@@ -88,13 +136,13 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   r.resize(n);
   clw_options opt{};
   opt.verbose = quiet ? 0u : 1u;
-  clw_session* S = nullptr;
-  if (clw_session_create(ctx, a.pmin, a.pmax, &opt, &S) != CLW_OK) {
+  Engine E;
+  if (E.create(ctx, a.pmin, a.pmax, n, &opt) != CLW_OK) {
     printf("error: %s\n", clw_last_error());
     return 1;
   }
   clw_range_stats info;
-  clw_session_info(S, &info);
+  E.info(&info);
   // read_state (cl_wilson.cpp:1736-1777): resume from the newest valid file.
   uint64_t currp = 2;
   bool write_a_next = true;
@@ -109,18 +157,15 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
     if (clw_ckpt_read(pa, pb, a.pmin, a.pmax, (uint32_t)n, &ws, lo.data(), hi.data(), &which, &res) !=
         CLW_OK) {
       printf("error: %s\n", clw_last_error());
-      clw_session_destroy(S);
       return 1;
     }
     if (res == 2) {
       printf("Workunit complete.\n");
-      clw_session_destroy(S);
       return 0;
     }
     if (res == 1) {
-      if (clw_session_set_residues(S, lo.data(), hi.data(), ws.totalcount) != CLW_OK) {
+      if (E.resume(lo.data(), hi.data(), n, ws.currp, ws.totalcount) != CLW_OK) {
         printf("error: %s\n", clw_last_error());
-        clw_session_destroy(S);
         return 1;
       }
       currp = ws.currp;
@@ -131,10 +176,8 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   }
   auto checkpoint = [&](uint64_t cp, uint32_t done) -> int {
     std::vector<uint64_t> lo(n), hi(n);
-    if (clw_session_residues(S, lo.data(), hi.data(), n) != CLW_OK) return 1;
-    clw_range_stats cur;
-    clw_session_info(S, &cur);
-    clw_work_status ws = {a.pmin, a.pmax, cp, (uint64_t)trickle, 0, cur.totalcount, (uint32_t)n, done};
+    if (E.residues(lo.data(), hi.data(), n) != CLW_OK) return 1;
+    clw_work_status ws = {a.pmin, a.pmax, cp, (uint64_t)trickle, 0, E.applied(), (uint32_t)n, done};
     char path[512];
     boinc_resolve_filename(write_a_next ? kStateA : kStateB, path, sizeof(path));
     if (clw_ckpt_write(path, &ws, lo.data(), hi.data()) == CLW_OK)
@@ -151,7 +194,7 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
   for (uint64_t qlo = currp; rc == CLW_OK && qlo <= info.maxtarget;) {
     const uint64_t qhi = std::min<uint64_t>(qlo + window, info.maxtarget + 1);
     boinc_begin_critical_section();
-    rc = clw_session_run(S, qlo, qhi);
+    rc = E.run(qlo, qhi);
     boinc_end_critical_section();
     qlo = qhi;
     boinc_fraction_done((double)qlo / (double)(info.maxtarget + 1));
@@ -160,10 +203,8 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
       if (!quiet) printf("Checkpoint, Current P: %" PRIu64 ", %.1f%% done\n", qlo, 100.0 * boinc_get_fraction_done());
     }
   }
-  if (rc == CLW_OK) rc = clw_session_iterate(S);
-  if (rc == CLW_OK) rc = clw_session_finalize(S, st, p.data(), q.data(), r.data(), n);
+  if (rc == CLW_OK) rc = E.finish(st, p.data(), q.data(), r.data(), n);
   if (rc == CLW_OK && !a.test) checkpoint(info.maxtarget + 1, 1);
-  clw_session_destroy(S);
   if (rc != CLW_OK) {
     fprintf(stderr, "error: %s\n", clw_last_error());
     printf("error: %s\n", clw_last_error());

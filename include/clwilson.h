@@ -368,6 +368,12 @@ int clw_batch_finish_range(clw_batch* b, uint32_t r, clw_range_stats* stats, uin
 uint64_t clw_batch_applied(const clw_batch* b);
 /* Canonical running residues of range r, in increasing p. */
 int clw_batch_residues(clw_batch* b, uint32_t r, uint64_t* lo, uint64_t* hi, uint64_t cap);
+/* Resume from a caller's checkpoint (the command line's stateA/B.ckp, read
+ * by clw_ckpt_read; cl_wilson.cpp:1736-1777 read_state): range r's residues
+ * in increasing p, holding the sieving primes [2, covered), `applied` of
+ * them. One-range batches, before their first window. */
+int clw_batch_set_residues(clw_batch* b, uint32_t r, const uint64_t* lo, const uint64_t* hi, uint64_t n,
+                           uint64_t covered, uint64_t applied);
 int clw_batch_iterate(clw_batch* b);
 int clw_batch_finalize(clw_batch* b);
 /* Range r after clw_batch_finalize or clw_batch_finish_range: stats

@@ -31,6 +31,49 @@ def test_resume_equals_uninterrupted(dev, tmp_path):
     assert out.totalcount == ref.totalcount
 
 
+def test_batch_resume_from_reference_checkpoint(dev, tmp_path):
+    """The command line runs a one-range batch and keeps the reference's
+    checkpoint files: residues written from a batch after [2, cut) and read
+    back into a new batch (clw_batch_set_residues) finish bit for bit like
+    an uninterrupted run, at 3 limbs through the matrix-core run products."""
+    lo, hi = 10**9, 10**9 + 200000
+    ref = dev.wilson_range(lo, hi)
+    a, b = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    cut = 123_456_789
+    with cw.Batch(dev, [(lo, hi)]) as bt:
+        bt.run(2, cut)
+        vals, applied = bt.residues(0), bt.applied()
+    with cw.Session(dev, lo, hi) as s:  # the file layout of the reference (cl_wilson.cpp:1779-1810)
+        s.set_residues(vals, applied)
+        s.checkpoint(a, cut)
+        n = s.n
+    res, which, st, back = cw.ckpt_read(a, b, lo, hi, n)
+    assert res == 1 and st.currp == cut and back == vals
+    with cw.Batch(dev, [(lo, hi)]) as bt:
+        bt.set_residues(0, back, st.currp, st.totalcount)
+        bt.run(st.currp, bt.maxtarget + 1)
+        bt.iterate()
+        bt.finalize()
+        out = bt.results(0)
+        total = bt.applied()
+    assert np.array_equal(out.quot, ref.quot) and out.checksum == ref.checksum
+    assert total == ref.totalcount
+
+
+def test_cli_large_range_through_the_batch(dev, tmp_path):
+    """A range of >= 65,536 test primes goes through a one-range batch on the
+    command line (matrix-core run products at 3 limbs): results.txt ends
+    with the session path's checksum."""
+    lo, hi = 10**9, 10**9 + 2_000_000
+    ref = dev.wilson_range(lo, hi, mode=1)
+    assert len(ref.p) >= 65536
+    exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
+    r = subprocess.run([exe, "-p", str(lo), "-P", str(hi)], cwd=tmp_path, capture_output=True, text=True, timeout=280)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(tmp_path / "results.txt").read().splitlines()
+    assert lines[-1] == ref.checksum_hex
+
+
 def test_cli_range_results_file_and_done_state(tmp_path):
     exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
     r = subprocess.run([exe, "-p", "200", "-P", "564"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
