@@ -380,7 +380,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would search its own "
-              "ranges; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "ranges, with no collective). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import clwilson_amd as cw
