@@ -153,7 +153,10 @@ def test_run_products_matrix_cores_equal_vector_units(dev):
     exponent-run boundaries (T/2 of 1e12's type-2 target 5e11, and type 1's
     target 2.5e11), T itself and the large-q region of 2e13."""
     ranges = [(10**12, 10**12 + 20000), (3 * 10**12, 3 * 10**12 + 20000), (2 * 10**13, 2 * 10**13 + 20000)]
-    windows = [(2, 10**6), (2 * 10**11 + 5 * 10**10 - 10**9, 2 * 10**11 + 5 * 10**10 + 10**9),
+    # (1e6, 2e6) is too short for the matrix cores and is the first window
+    # replayed from a captured graph; the next one allocates the byte tables,
+    # so the step is captured again with them
+    windows = [(2, 10**6), (10**6, 2 * 10**6), (2 * 10**11 + 5 * 10**10 - 10**9, 2 * 10**11 + 5 * 10**10 + 10**9),
                (5 * 10**11 - 10**9, 5 * 10**11 + 10**9), (10**12, 10**12 + 2 * 10**9)]
     got = {}
     for mode in (0, 1, 2):
