@@ -173,6 +173,22 @@ def test_run_products_matrix_cores_equal_vector_units(dev):
     assert got[1][1]["mfma_int8_macs"] == 0
 
 
+def test_run_products_matrix_cores_top_of_band(dev):
+    """Test primes just below the 3-limb band's top (2 p^2 close to 2^90):
+    k_mfma_runs keeps mul_full's conditional subtraction there (the lazy
+    product needs N < 0.4998 R) and still equals the vector reduction."""
+    ranges = [(24_876_000_000_000, 24_876_000_020_000)]
+    qlo, qhi = 10**12, 10**12 + 2 * 10**9
+    got = {}
+    for mode in (0, 1):
+        with cw.Batch(dev, ranges, run_products=mode) as b:
+            assert b.info[0].limbs == 3 and 2 * (ranges[0][1] - 1) ** 2 > 0.9996 * 2**90
+            b.run(qlo, qhi)
+            got[mode] = (b.residues(0), b.kernel_stats())
+    assert got[0][0] == got[1][0]
+    assert got[0][1]["mfma_int8_macs"] > 0 and got[1][1]["mfma_int8_macs"] == 0
+
+
 def test_batch_rejects_bad_ranges(dev):
     with pytest.raises(cw.ClwError):
         cw.Batch(dev, [(10, 10)])
