@@ -154,6 +154,7 @@ def lib():
         "clw_finalize": (i32, [vp, P(TpTable), vp, vp, vp]),
         "clw_wilson_range": (i32, [vp, u64, u64, P(Options), P(RangeStats), vp, vp, vp, u64]),
         "clw_format_result": (i32, [u64, u64, C.c_char_p, C.c_size_t]),
+        "clw_verify_results": (i32, [C.c_char_p, u64, u64, u64, vp, vp, u64, P(u32), P(u32)]),
         "clw_range_types": (i32, [u64, u64, P(u32), P(u64)]),
         "clw_host_primes": (u64, [u64, u64, vp, u64]),
         "clw_mul_records_lanes": (i32, [vp, P(TpTable), P(SegBuffers), u32, u32, u32, u32, u32, vp, vp, u32]),
@@ -237,6 +238,20 @@ def host_primes(lo: int, hi: int) -> np.ndarray:
 def format_result(p: int, quot: int) -> str | None:
     buf = C.create_string_buffer(128)
     return buf.value.decode() if lib().clw_format_result(p, quot, buf, 128) else None
+
+
+GOOD_RESULTS_SUM = 0x6659912234F44428  # the reference's goodWilsonResults.txt (cl_wilson.cpp:970)
+
+
+def verify_results(path: str, pmin: int, pmax: int, p, quot, expected_sum: int = GOOD_RESULTS_SUM) -> tuple[int, int]:
+    """The -r check (clw_verify_results): (matched, listed) on success;
+    raises ClwError (CLW_ERR_CHECK / CLW_ERR_IO) with the reference's message."""
+    pa = np.ascontiguousarray(p, dtype=np.uint64)
+    qa = np.ascontiguousarray(quot, dtype=np.uint64)
+    m, n = C.c_uint32(0), C.c_uint32(0)
+    check(lib().clw_verify_results(path.encode(), expected_sum, pmin, pmax, pa.ctypes.data, qa.ctypes.data, len(pa),
+                                   C.byref(m), C.byref(n)), "verify_results")
+    return m.value, n.value
 
 
 def range_types(pmin: int, pmax: int) -> tuple[list[int], list[int]]:

@@ -161,6 +161,62 @@ extern "C" int clw_format_result(uint64_t p, uint64_t quot, char* buf, size_t bu
   return 0;
 }
 
+// Result verification against a known-good list (cl_wilson.cpp:902-930,
+// 936-1014, 1073-1087).
+extern "C" int clw_verify_results(const char* path, uint64_t expected_sum, uint64_t pmin, uint64_t pmax,
+                                  const uint64_t* p, const uint64_t* quot, uint64_t n, uint32_t* matched,
+                                  uint32_t* listed) {
+  if (!path || (n && (!p || !quot))) return clw_fail(CLW_ERR_ARG, "clw_verify_results: null argument");
+  if (matched) *matched = 0;
+  if (listed) *listed = 0;
+  FILE* f = fopen(path, "r");
+  if (!f) return clw_fail(CLW_ERR_IO, "Cannot open %s !!!", path);
+  std::vector<std::pair<uint64_t, int32_t>> good;  // the entries in [pmin, pmax)
+  uint64_t sum = 0;
+  char line[256];
+  while (fgets(line, sizeof(line), f)) {
+    unsigned long long gp = 0;
+    int gv = 0;
+    if (sscanf(line, "%llu %d", &gp, &gv) != 2) {
+      fclose(f);
+      return clw_fail(CLW_ERR_IO, "Error reading %s !!!", path);
+    }
+    sum += (uint64_t)gp + (uint64_t)(int64_t)gv;  // uint64 + int32, as cl_wilson.cpp:954
+    if (gp >= pmin && gp < pmax) good.push_back({(uint64_t)gp, (int32_t)gv});
+  }
+  fclose(f);
+  if (sum != expected_sum) return clw_fail(CLW_ERR_CHECK, "Checksum error in %s !!!", path);
+  if (listed) *listed = (uint32_t)good.size();
+  uint32_t nm = 0;
+  const double threshold = 1.0 / 50000.0;
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint64_t neg = p[k] - quot[k], smallest = quot[k] <= neg ? quot[k] : neg;
+    // the ratio rounded to double before the compare, as cl_wilson.cpp:906-907
+    const double wpp = (double)((long double)smallest / (long double)p[k]);
+    if (!(wpp < threshold)) continue;
+    const int32_t dq = smallest == quot[k] ? (int32_t)quot[k] : -(int32_t)neg;
+    bool found = false;
+    for (auto& e : good) {
+      if (e.first != p[k]) continue;
+      if (e.second != dq)
+        return clw_fail(CLW_ERR_CHECK, "result check failed! p: %llu %d, result file is %d",
+                        (unsigned long long)p[k], dq, e.second);
+      e.first = 0;  // matched once
+      found = true;
+      ++nm;
+      break;
+    }
+    if (!found)
+      return clw_fail(CLW_ERR_CHECK, "result check failed! p: %llu is not in result file!", (unsigned long long)p[k]);
+  }
+  if (matched) *matched = nm;
+  for (const auto& e : good)
+    if (e.first)
+      return clw_fail(CLW_ERR_CHECK, "result check failed! p: %llu was not found in results!",
+                      (unsigned long long)e.first);
+  return CLW_OK;
+}
+
 namespace {
 
 #define CK(x)                        \
@@ -640,8 +696,14 @@ extern "C" int clw_wilson_range(clw_ctx* ctx, uint64_t pmin, uint64_t pmax, cons
 // ---------------------------------------------------------------------------
 // Checkpoints (cl_wilson.cpp:110-319)
 // ---------------------------------------------------------------------------
+// The reference's additive sum (cl_wilson.cpp:116-119) plus a format
+// constant: this library stores canonical residues where the reference stores
+// Montgomery digits, so a file of one program must never validate in the
+// other (the residues would be read in the wrong form).
+static const uint64_t kCkptFormatSalt = 0x434C57414D443031ull;  // "CLWAMD01"
 static uint64_t ckpt_sum(const clw_work_status* st, const uint64_t* lo, const uint64_t* hi) {
-  uint64_t s = st->pmin + st->pmax + st->currp + st->trickle + st->totalcount + st->tpcount + st->done;
+  uint64_t s = kCkptFormatSalt + st->pmin + st->pmax + st->currp + st->trickle + st->totalcount + st->tpcount +
+               st->done;
   for (uint32_t i = 0; i < st->tpcount; ++i) s += lo[i] + hi[i];
   return s;
 }
@@ -684,9 +746,11 @@ int ckpt_read_one(const char* path, uint64_t pmin, uint64_t pmax, uint32_t tpcou
   }
   fclose(f);
   if (!r) return 0;
-  if (st->done) return 2;  // cl_wilson.cpp:192-194: a finished workunit
+  // the sum is checked for finished files too (the reference returns 2
+  // before its check, cl_wilson.cpp:192-194): a foreign or damaged file
+  // never ends a workunit
   if (ckpt_sum(st, lo.data(), hi.data()) != st->state_sum) return 0;
-  return 1;
+  return st->done ? 2 : 1;
 }
 }  // namespace
 

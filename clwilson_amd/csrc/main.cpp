@@ -23,6 +23,7 @@
 static const uint64_t kMaxP = 0xFFFFFFFFFFFFFFFFull / 4;  // cl_wilson.h:9
 static const char* kResultFile = "results.txt";
 static const char* kGoodFile = "goodWilsonResults.txt";
+static const uint64_t kGoodFileSum = 0x6659912234F44428ull;  // cl_wilson.cpp:970
 static const char* kStateA = "stateA.ckp";  // cl_wilson.h:5-6
 static const char* kStateB = "stateB.ckp";
 
@@ -49,8 +50,9 @@ struct Args {
 // kBatchMinTestPrimes test primes below 8.2e17, as clw_wilson_range chooses;
 // else a session (a batch builds every sub-chunk product of a segment, which
 // only pays off over many test primes; 5 limbs). Both keep one canonical
-// residue per test prime in increasing p, so the reference's checkpoint
-// files serve either.
+// residue per test prime in increasing p, so this program's checkpoint files
+// (the reference's names and header; canonical residues and a salted sum,
+// so files of the reference itself are rejected) serve either.
 static const uint64_t kBatchMinTestPrimes = 65536;
 struct Engine {
   clw_session* S = nullptr;
@@ -91,6 +93,51 @@ struct Engine {
     if (S) clw_session_destroy(S);
   }
 };
+
+// The -r verification (cl_wilson.cpp:902-1014, 1073-1087), then
+// results.txt: one line per (near-)Wilson prime, then the checksum
+// (cl_wilson.cpp:657-688, 1093-1143). 0 when both succeeded.
+static int finish_outputs(const Args& a, const std::vector<uint64_t>& p, const std::vector<uint64_t>& q, uint64_t n,
+                          const clw_range_stats& st, bool quiet) {
+  if (a.result_test) {
+    uint32_t matched = 0, listed = 0;
+    if (clw_verify_results(kGoodFile, kGoodFileSum, a.pmin, a.pmax, p.data(), q.data(), n, &matched, &listed) !=
+        CLW_OK) {
+      fprintf(stderr, "error: %s\n", clw_last_error());
+      printf("error: %s\n", clw_last_error());
+      return 1;
+    }
+    printf("Read %u good results |w_p/p| < 1/50000 for search range from %s\n", listed, kGoodFile);
+    printf("All results matched result file!\n");
+  }
+  char rpath[512];
+  boinc_resolve_filename(kResultFile, rpath, sizeof(rpath));
+  FILE* out = boinc_fopen(rpath, "w");
+  if (!out) {
+    printf("Cannot open %s !!!\n", kResultFile);
+    return 1;
+  }
+  char line[128];
+  uint32_t lines = 0;
+  bool ok = true;
+  for (uint64_t k = 0; k < n; ++k) {
+    if (clw_format_result(p[k], q[k], line, sizeof(line))) {
+      ok = ok && fputs(line, out) >= 0;
+      ++lines;
+      if (!a.result_test && !quiet) fputs(line, stdout);
+    }
+  }
+  if (lines == 0)
+    ok = ok && fprintf(out, "no results\n%016" PRIX64 "\n", st.checksum) >= 0;
+  else
+    ok = ok && fprintf(out, "%016" PRIX64 "\n", st.checksum) >= 0;
+  if (fclose(out) != 0) ok = false;
+  if (!ok) {
+    printf("Cannot write to %s !!!\n", kResultFile);
+    return 1;
+  }
+  return 0;
+}
 
 static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quiet) {
   // setupSearch (cl_wilson.cpp:1146-1182)
@@ -204,91 +251,22 @@ static int run_range(clw_ctx* ctx, const Args& a, clw_range_stats* st, bool quie
     }
   }
   if (rc == CLW_OK) rc = E.finish(st, p.data(), q.data(), r.data(), n);
-  if (rc == CLW_OK && !a.test) checkpoint(info.maxtarget + 1, 1);
   if (rc != CLW_OK) {
     fprintf(stderr, "error: %s\n", clw_last_error());
     printf("error: %s\n", clw_last_error());
     return 1;
   }
-  // results.txt: one line per (near-)Wilson prime, then the checksum
-  // (cl_wilson.cpp:657-688, 1093-1143).
-  char rpath[512];
-  boinc_resolve_filename(kResultFile, rpath, sizeof(rpath));
-  FILE* out = boinc_fopen(rpath, "w");
-  if (!out) {
-    printf("Cannot open %s !!!\n", kResultFile);
-    return 1;
-  }
-  char line[128];
-  uint32_t lines = 0;
-  for (uint64_t k = 0; k < n; ++k) {
-    if (clw_format_result(p[k], q[k], line, sizeof(line))) {
-      fputs(line, out);
-      ++lines;
-      if (!a.result_test && !quiet) fputs(line, stdout);
-    }
-  }
-  if (lines == 0)
-    fprintf(out, "no results\n%016" PRIX64 "\n", st->checksum);
-  else
-    fprintf(out, "%016" PRIX64 "\n", st->checksum);
-  if (fclose(out) != 0) {
-    printf("Cannot close %s !!!\n", kResultFile);
-    return 1;
-  }
-  if (a.result_test) {
-    // cl_wilson.cpp:902-1014: every p with |w_p/p| < 1/50000 must be listed.
-    FILE* g = fopen(kGoodFile, "r");
-    if (!g) {
-      printf("Cannot open %s !!!\n", kGoodFile);
-      return 1;
-    }
-    std::vector<std::pair<uint64_t, int32_t>> good;
-    uint64_t sum = 0, gp;
-    int32_t gv;
-    char buf[256];
-    while (fgets(buf, sizeof(buf), g)) {
-      if (sscanf(buf, "%" SCNu64 " %" SCNd32, &gp, &gv) != 2) {
-        printf("Error reading %s !!!\n", kGoodFile);
-        fclose(g);
-        return 1;
-      }
-      sum += gp + (uint64_t)(int64_t)gv;
-      if (gp >= a.pmin && gp < a.pmax) good.push_back({gp, gv});
-    }
-    fclose(g);
-    if (sum != 0x6659912234F44428ull) {
-      printf("Checksum error in %s !!!\n", kGoodFile);
-      return 1;
-    }
-    size_t matched = 0;
-    for (uint64_t k = 0; k < n; ++k) {
-      const uint64_t neg = p[k] - q[k], small = q[k] <= neg ? q[k] : neg;
-      if ((long double)small / (long double)p[k] >= 1.0L / 50000.0L) continue;
-      const int32_t dq = small == q[k] ? (int32_t)q[k] : -(int32_t)neg;
-      bool ok = false;
-      for (auto& e : good)
-        if (e.first == p[k]) {
-          if (e.second != dq) {
-            printf("error: result check failed! p: %" PRIu64 " %d, result file is %d\n", p[k], dq, e.second);
-            return 1;
-          }
-          e.first = 0;
-          ok = true;
-          ++matched;
-          break;
-        }
-      if (!ok) {
-        printf("error: result check failed! p: %" PRIu64 " is not in result file!\n", p[k]);
-        return 1;
-      }
-    }
-    if (matched != good.size()) {
-      printf("error: result check failed! %zu results of the file were not found\n", good.size() - matched);
-      return 1;
-    }
-    printf("All results matched result file!\n");
-  }
+  // As getResults + finalizeResults + the done checkpoint of
+  // cl_wilson.cpp:1949-1956, in one critical section: the -r check, then
+  // results.txt (every write checked), and only once it is closed the
+  // checkpoint that marks the workunit done, so a failure or a kill before
+  // that point leaves a workunit that is searched again, never one that
+  // reports complete without its results.
+  boinc_begin_critical_section();
+  const int out_rc = finish_outputs(a, p, q, n, *st, quiet);
+  if (out_rc == 0 && !a.test) checkpoint(info.maxtarget + 1, 1);
+  boinc_end_critical_section();
+  if (out_rc != 0) return out_rc;
   if (!quiet) {
     printf("Search finished in %.2f sec.\n", st->seconds);
     printf("results %u, total power table primes generated %" PRIu64 ", checksum %016" PRIX64 "\n", st->resultcount,

@@ -420,10 +420,15 @@ int clw_batch_kernel_stats(const clw_batch* b, clw_kernel_stats* out);
 /* ---- checkpoint / resume (cl_wilson.cpp:110-319, 1734-1777) ------------- */
 
 /* The reference's checkpoint header (workStatus, cl_wilson.h:22-25), 56 bytes,
- * followed in the file by tpcount (lo, hi) residue pairs. This build stores
- * each residue in canonical form (value mod p^2, increasing p) where the
- * reference stores Montgomery digits. state_sum = pmin + pmax + currp +
- * trickle + totalcount + tpcount + done + sum of all residue words (mod 2^64). */
+ * followed in the file by tpcount (lo, hi) residue pairs, under the
+ * reference's file names. The files are not interchangeable with the
+ * reference's: this build stores each residue in canonical form (value mod
+ * p^2, increasing p) where the reference stores Montgomery digits, so its
+ * sum carries a format constant the reference's lacks (either program
+ * rejects the other's files and starts over). state_sum = 0x434C57414D443031
+ * ("CLWAMD01") + pmin + pmax + currp + trickle + totalcount + tpcount + done
+ * + sum of all residue words (mod 2^64); the reference's sum has no constant
+ * (cl_wilson.cpp:116-119). */
 typedef struct {
   uint64_t pmin, pmax, currp, trickle, state_sum, totalcount;
   uint32_t tpcount, done;
@@ -433,8 +438,9 @@ typedef struct {
  * filling st->state_sum. */
 int clw_ckpt_write(const char* path, clw_work_status* st, const uint64_t* res_lo, const uint64_t* res_hi);
 /* read_state (cl_wilson.cpp:159-288): reads both files, rejects a file that
- * cannot be parsed, belongs to another range/count, or fails its checksum,
- * and picks the one with the larger currp. *result = 0 (no valid file),
+ * cannot be parsed, belongs to another range/count, or fails its checksum
+ * (finished files included, unlike the reference), and picks the one with
+ * the larger currp. *result = 0 (no valid file),
  * 1 (resume: st and residues filled, *which = 0 for A / 1 for B),
  * 2 (a valid file says done). res_* need room for tpcount entries. */
 int clw_ckpt_read(const char* path_a, const char* path_b, uint64_t pmin, uint64_t pmax, uint32_t tpcount,
@@ -446,6 +452,19 @@ int clw_session_set_residues(clw_session* s, const uint64_t* lo, const uint64_t*
 /* Result line of cl_wilson.cpp:657-688 for one test prime into buf; returns
  * 1 if the prime is reported (quot = 0 or min(quot, p-quot) < 1000), else 0. */
 int clw_format_result(uint64_t p, uint64_t quot, char* buf, size_t buflen);
+
+/* Result verification (the -r option; cl_wilson.cpp:902-930, 936-1014,
+ * 1073-1087, readGoodResultFile + the resultTest branch of processResult):
+ * reads a known-good list of "p w_p" lines, checks that the additive sum of
+ * every line's p + w_p (mod 2^64) equals expected_sum (the reference's file:
+ * 0x6659912234F44428), then requires every tested prime with |w_p/p| <
+ * 1/50000 (the ratio rounded to double, as the reference) to be listed with
+ * the same w_p, and every listed p in [pmin, pmax) to have been tested and
+ * matched. CLW_OK, CLW_ERR_IO (missing or unreadable file) or CLW_ERR_CHECK
+ * (checksum or match failure; the reference's message in clw_last_error).
+ * matched / listed (may be NULL): primes matched, lines in [pmin, pmax). */
+int clw_verify_results(const char* path, uint64_t expected_sum, uint64_t pmin, uint64_t pmax, const uint64_t* p,
+                       const uint64_t* quot, uint64_t n, uint32_t* matched, uint32_t* listed);
 
 /* Counts of test primes by type and the type targets of [pmin, pmax)
  * (cl_wilson.cpp:1552-1586), computed on the host. */

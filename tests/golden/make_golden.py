@@ -1,6 +1,7 @@
 """Writes the golden fixtures of tests/golden/ with the CPU oracle.
 
     python tests/golden/make_golden.py [--cfg1] [--direct]
+    python tests/golden/make_golden.py --mfma | --cfg2 | --cfg4   (one fixture each)
 
 Fixtures (JSON):
   kat_selftest.json   the reference's seven self-test answers (cl_wilson.cpp:2007-2123)
@@ -13,6 +14,16 @@ Fixtures (JSON):
                       (p-1)! = 2 * 3 * ... * (p-1) mod p^2: per range [5, 1e6)
                       and [1e6, 2e6) the count, checksum, sha256 of the
                       quotient vector, and every 500th (p, quot) as samples
+  mfma_windows.json   (--mfma) window products at p ~ 2e13 and at the top of
+                      the 3-limb band (2.4876e13), each window ~1.4 M sieving
+                      primes (long enough for the matrix-core run products)
+                      across exponent-run boundaries T/2, T/3 and targets
+  cfg2_oracle.json    (--cfg2) the full plain method for 16 sampled primes of
+                      [1e11, 1e11+1e7): product of every sieving prime up to
+                      the type target, iterate, final check -> quot
+  cfg4_windows.json   (--cfg4) window products of sampled test primes of the
+                      256-range batch (workloads.batch_ranges(), seed 1234)
+                      over two windows of sieving primes
 """
 from __future__ import annotations
 
@@ -118,9 +129,121 @@ def acu_values(p, t):
     return 0, 0
 
 
+def type_targets(ps) -> tuple[list[int], dict[int, int]]:
+    """Types of the test primes of one range and each type's target, taken
+    from the range's first prime of that type (cl_wilson.cpp:1552-1586)."""
+    typ = [0 if int(p) % 3 == 1 else (1 if int(p) % 12 == 5 else 2) for p in ps]
+    T = {}
+    for p, t in zip(ps, typ):
+        T.setdefault(t, (int(p) - 1) // (6, 4, 2)[t])
+    return typ, T
+
+
+MFMA_CASES = [  # (pmin, pmax, [window centre: an exponent-run boundary or a target])
+    (20_000_000_000_000, 20_000_000_001_200, ["T2/2", "T2/3", "T1/2", "T0-"]),
+    (24_876_000_000_000, 24_876_000_001_200, ["T2/2", "T0-"]),
+]
+MFMA_HALF = 20_000_000  # windows of 4e7 integers (~1.4 M primes, ~10,800 sub-chunks)
+
+
+def mfma_windows():
+    out = []
+    for plo, phi, centres in MFMA_CASES:
+        ps = O.primes_range(plo, phi)
+        typ, T = type_targets(ps)
+        targets = np.array([T[t] for t in typ], dtype=np.uint64)
+        for c in centres:
+            if c.endswith("-"):  # the window ends at the type-0 target
+                qhi = T[0] + 1
+                qlo = qhi - 2 * MFMA_HALF
+            else:
+                t, d = int(c[1]), int(c[3])
+                m = T[t] // d
+                qlo, qhi = m - MFMA_HALF, m + MFMA_HALF
+            t0 = time.time()
+            vals = O.window_product(ps, targets, qlo, qhi)
+            print("mfma window", plo, c, qlo, qhi, round(time.time() - t0, 1), flush=True)
+            out.append(dict(pmin=plo, pmax=phi, at=c, qlo=qlo, qhi=qhi, p=[int(x) for x in ps],
+                            type_targets={str(k): v for k, v in T.items()}, value=[str(v) for v in vals]))
+    dump("mfma_windows.json", out)
+
+
+def _cfg2_window(args):
+    ps, targets, a, b = args
+    return O.window_product(np.array(ps, dtype=np.uint64), np.array(targets, dtype=np.uint64), a, b, nthreads=1)
+
+
+CFG2 = (100_000_000_000, 100_010_000_000)
+
+
+def cfg2_samples(nsample=16, width=500_000_000, workers=8):
+    from multiprocessing import Pool
+    lo, hi = CFG2
+    allp = O.primes_range(lo, hi)
+    typ, T = type_targets(allp)
+    rng = np.random.default_rng(2)
+    idx = sorted(set([0, len(allp) - 1] + [int(i) for i in rng.choice(len(allp), nsample - 2, replace=False)]))
+    ps = [int(allp[i]) for i in idx]
+    ty = [typ[i] for i in idx]
+    targets = [T[t] for t in ty]
+    maxT = max(targets)
+    jobs = [(ps, targets, a, min(a + width, maxT + 1)) for a in range(2, maxT + 1, width)]
+    t0 = time.time()
+    acc = [1] * len(ps)
+    with Pool(workers) as pool:
+        for k, vals in enumerate(pool.imap(_cfg2_window, jobs)):
+            acc = [x * v % (p * p) for x, v, p in zip(acc, vals, ps)]
+            if k % 10 == 0:
+                print("cfg2 window", k, "of", len(jobs), round(time.time() - t0), "s", flush=True)
+    own = np.array([(p - 1) // (6, 4, 2)[t] for p, t in zip(ps, ty)], dtype=np.uint64)
+    itv = O.iterate(np.array(ps, dtype=np.uint64), np.array(targets, dtype=np.uint64), own, acc)
+    rows = []
+    for i, p, t, v in zip(idx, ps, ty, itv):
+        acu1, acu2 = acu_values(p, t)
+        q, r = O.finalize(p, t, v, acu1, acu2)
+        assert r == 0, (p, r)
+        rows.append(dict(index=i, p=p, type=t, quot=q, w=O.wilson_quotient(p, q)))
+    dump("cfg2_oracle.json", dict(pmin=lo, pmax=hi, tpcount=len(allp),
+                                  type_targets={str(k): v for k, v in T.items()}, samples=rows))
+    print("cfg2 samples", round(time.time() - t0), "s", flush=True)
+
+
+CFG4_RANGES = [0, 30, 60, 100, 140, 180, 200, 215, 221, 222, 240, 255]  # 222 on: the 4-limb band
+CFG4_WINDOWS = [(1_000_000_000, 1_040_000_000), (2_000_000_000_000, 2_000_040_000_000)]
+
+
+def cfg4_windows():
+    sys.path.insert(0, os.path.join(HERE, "..", ".."))
+    from clwilson_amd.workloads import batch_ranges
+    ranges = batch_ranges()
+    rows = []
+    for r in CFG4_RANGES:
+        lo, hi = ranges[r]
+        allp = O.primes_range(lo, hi)
+        typ, T = type_targets(allp)
+        n = len(allp)
+        idx = sorted(set([0, 1, 2, n // 3, n // 2, n - 2, n - 1]))
+        ps = np.array([allp[i] for i in idx], dtype=np.uint64)
+        targets = np.array([T[typ[i]] for i in idx], dtype=np.uint64)
+        vals = []
+        for qlo, qhi in CFG4_WINDOWS:
+            t0 = time.time()
+            vals.append([str(v) for v in O.window_product(ps, targets, qlo, qhi)])
+            print("cfg4 range", r, qlo, round(time.time() - t0, 1), flush=True)
+        rows.append(dict(range=r, pmin=lo, pmax=hi, tpcount=n, index=idx, p=[int(x) for x in ps],
+                         targets=[int(x) for x in targets], windows=vals))
+    dump("cfg4_windows.json", dict(windows=CFG4_WINDOWS, ranges=rows))
+
+
 def main():
     if sys.argv[1:] == ["--direct"]:
         return direct_only()
+    if sys.argv[1:] == ["--mfma"]:
+        return mfma_windows()
+    if sys.argv[1:] == ["--cfg2"]:
+        return cfg2_samples()
+    if sys.argv[1:] == ["--cfg4"]:
+        return cfg4_windows()
     dump("kat_selftest.json", [dict(pmin=a, pmax=b, results=c, checksum=f"{d:016X}", pi_maxt=e, lastp=f,
                                     lastw=g) for a, b, c, d, e, f, g in KATS])
     dump("cgh_tables.json", dict(table1=CGH_T1, table2=CGH_T2))

@@ -22,7 +22,8 @@ def test_layout_and_checksum(tmp_path):
     pmin, pmax, currp, trickle, ssum, total, tpc, done = struct.unpack("<6Q2I", raw[:56])
     assert (pmin, pmax, currp, trickle, total, tpc, done) == (100, 200, 150, 7, 42, 5, 0)
     words = struct.unpack("<10Q", raw[56:])
-    expect = (100 + 200 + 150 + 7 + 42 + 5 + 0 + sum(words)) % 2**64
+    # the reference's sum (cl_wilson.cpp:116-119) plus the format constant "CLWAMD01"
+    expect = (0x434C57414D443031 + 100 + 200 + 150 + 7 + 42 + 5 + 0 + sum(words)) % 2**64
     assert ssum == expect
     assert [words[2 * i] | (words[2 * i + 1] << 64) for i in range(5)] == vals
 
@@ -56,3 +57,30 @@ def test_done_flag(tmp_path):
     cw.ckpt_write(a, cw.WorkStatus(10, 20, 300, 0, 0, 9, 4, 0), _vals(4))
     cw.ckpt_write(b, cw.WorkStatus(10, 20, 300, 0, 0, 9, 4, 1), _vals(4))
     assert cw.ckpt_read(a, b, 10, 20, 4)[0] == 2
+
+
+def test_reference_written_files_are_rejected(tmp_path):
+    """A file with the reference's own checksum (cl_wilson.cpp:116-119: the
+    plain sum, over Montgomery-form residues) for the same range and count
+    never validates here, done or not: its residues are in another form."""
+    a, b = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    vals = _vals(4, 3)
+    for done in (0, 1):
+        words = [w for v in vals for w in (v & (2**64 - 1), v >> 64)]
+        ssum = (10 + 20 + 300 + 0 + 9 + 4 + done + sum(words)) % 2**64
+        raw = struct.pack("<6Q2I", 10, 20, 300, 0, ssum, 9, 4, done) + struct.pack(f"<{len(words)}Q", *words)
+        open(a, "wb").write(raw)
+        assert cw.ckpt_read(a, b, 10, 20, 4)[0] == 0
+    # this library's own file of the same content is accepted
+    cw.ckpt_write(a, cw.WorkStatus(10, 20, 300, 0, 0, 9, 4, 0), vals)
+    res, which, st, got = cw.ckpt_read(a, b, 10, 20, 4)
+    assert (res, which, got) == (1, 0, vals)
+
+
+def test_done_file_with_a_bad_sum_is_not_done(tmp_path):
+    a, b = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    cw.ckpt_write(a, cw.WorkStatus(10, 20, 300, 0, 0, 9, 4, 1), _vals(4))
+    raw = bytearray(open(a, "rb").read())
+    raw[-1] ^= 0x40
+    open(a, "wb").write(bytes(raw))
+    assert cw.ckpt_read(a, b, 10, 20, 4)[0] == 0

@@ -4,7 +4,6 @@ the reference's self-test answers and the CGH published Wilson quotients.
 Integer work, so every comparison is bit-exact.
 """
 import hashlib
-import os
 import sys
 
 import numpy as np
@@ -169,26 +168,6 @@ def test_selftest_large_primes(dev, golden, k):
 def test_device_primes_equal_host_primes(dev, lo, hi):
     """The device-sieved test-prime list equals the host segmented sieve."""
     assert np.array_equal(dev.primes(lo, hi), cw.host_primes(lo, hi))
-
-
-@pytest.mark.slow
-@pytest.mark.skipif(not os.environ.get("CLW_LONG"), reason="full config 2 takes ~3.5 min per path; set CLW_LONG=1")
-def test_config2_full_range(dev, golden):
-    """[1e11, 1e11+1e7), 394,401 test primes, full search through the batched
-    engine: checksum, quotient hash and pi(maxtarget) (5e10) as recorded."""
-    g = golden("cfg2_gpu.json")
-    with cw.Batch(dev, [(g["pmin"], g["pmax"])]) as b:
-        M = b.maxtarget
-        for a in range(2, M + 1, 5_000_000_000):
-            b.run(a, min(a + 5_000_000_000, M + 1))
-            say("covered", b.covered())
-        applied = b.applied()
-        b.iterate()
-        b.finalize()
-        r = b.results(0)
-    assert applied == g["pi_maxt"]  # every prime up to the largest target, once
-    assert len(r.p) == g["tpcount"] and r.checksum_hex == g["checksum"] and r.results == g["results"]
-    assert hashlib.sha256(np.ascontiguousarray(r.quot, dtype="<u8").tobytes()).hexdigest() == g["quot_sha256"]
 
 
 @pytest.mark.parametrize("engine", ["session", "batch"])

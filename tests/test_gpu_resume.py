@@ -94,3 +94,46 @@ def test_cli_suffix_arguments(tmp_path):
     import oracle as O
     arr, st = O.wilson_range(86000000, 86010000)
     assert open(tmp_path / "results.txt").read().splitlines()[-1] == f"{st['checksum']:016X}"
+
+
+def test_cli_failed_results_write_is_not_complete(tmp_path):
+    """results.txt that cannot be written (a directory of that name): the run
+    fails and writes no finished checkpoint, so the rerun searches again and
+    writes the results instead of reporting 'Workunit complete.'."""
+    exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
+    os.mkdir(tmp_path / "results.txt")
+    r = subprocess.run([exe, "-p", "200", "-P", "564"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0, r.stdout + r.stderr
+    os.rmdir(tmp_path / "results.txt")
+    r = subprocess.run([exe, "-p", "200", "-P", "564"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Workunit complete." not in r.stdout
+    assert open(tmp_path / "results.txt").read().splitlines()[-1] == "00000000000080A3"
+
+
+def _good_file(path, rows):
+    """A known-good list whose p + w sum equals the reference file's constant
+    (one balancing line far outside any searched range)."""
+    s = sum(p + w for p, w in rows) % 2**64
+    rows = rows + [((cw.GOOD_RESULTS_SUM - s) % 2**64, 0)]
+    with open(path, "w") as f:
+        for p, w in rows:
+            f.write(f"{p} {w}\n")
+
+
+def test_cli_result_verification(tmp_path):
+    """-r (cl_wilson.cpp:902-1014): 563 is the only prime of [200, 564) with
+    |w_p/p| < 1/50000 (w = 0); a file listing it passes, one with another
+    w_p fails and leaves the workunit unfinished."""
+    exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
+    _good_file(tmp_path / "goodWilsonResults.txt", [(5, 0), (13, 0), (563, 0), (8315831, 3)])
+    r = subprocess.run([exe, "-p", "200", "-P", "564", "-r"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "All results matched result file!" in r.stdout
+    d = tmp_path / "bad"
+    d.mkdir()
+    _good_file(d / "goodWilsonResults.txt", [(563, 1)])
+    r = subprocess.run([exe, "-p", "200", "-P", "564", "-r"], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "result check failed! p: 563" in r.stdout
+    assert not os.path.exists(d / "results.txt")
