@@ -175,6 +175,7 @@ def lib():
         "clw_plain_reduce": (i32, [vp, P(TpTable), u32, u32, vp, vp]),
         "clw_iterate_from": (i32, [vp, P(TpTable), vp]),
         "clw_device_primes": (i32, [vp, u64, u64, vp, u64, P(u64)]),
+        "clw_device_primes_noalloc": (i32, [vp, u64, u64, vp, u64, P(u64)]),
         "clw_batch_create": (i32, [vp, vp, vp, u32, P(Options), P(vp)]),
         "clw_batch_destroy": (i32, [vp]),
         "clw_batch_size": (u32, [vp]),
@@ -320,6 +321,16 @@ class Device:
         n = C.c_uint64(0)
         check(lib().clw_device_primes(self.ctx, lo, hi, out.ctypes.data, cap, C.byref(n)), "device_primes")
         return out[: n.value]
+
+    def primes_noalloc(self, lo: int, hi: int, cap: int | None = None) -> tuple[np.ndarray, int]:
+        """The same list through clw_device_primes_noalloc (no device
+        allocation; hi - lo <= 10^7): (primes written, total count)."""
+        cap = (hi - lo) // 2 + 4 if cap is None else cap
+        out = np.zeros(max(cap, 1), dtype=np.uint64)
+        n = C.c_uint64(0)
+        check(lib().clw_device_primes_noalloc(self.ctx, lo, hi, out.ctypes.data, cap, C.byref(n)),
+              "device_primes_noalloc")
+        return out[: min(cap, n.value)], n.value
 
     def wilson_range(self, pmin: int, pmax: int, segment_span: int = 0, target_waves: int = 0,
                      verbose: bool = False, allow_check_failure: bool = False, mode: int = 0,

@@ -26,6 +26,9 @@ int clw_check_launch(const char* kernel);
 static inline int clw_hip(hipError_t e, const char* what) { return e == hipSuccess ? CLW_OK : clw_fail_hip(e, what); }
 static inline hipStream_t clw_stream(clw_ctx* ctx) { return ctx->stream; }
 int clw_upload_sieve_tables();
+// Looks up the current device's address of the allocation-free entry point's
+// scratch array (clw_kernels.hip), once per device; called by clw_init.
+int clwi_noalloc_setup(int device);
 
 // Run products of the batched engine: a sub-chunk is kBatchSubRecs pair
 // records (128 sieving primes), at most kBatchSubCap per sieve block, each

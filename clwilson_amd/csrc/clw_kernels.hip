@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "clw_arith.h"
@@ -940,6 +941,99 @@ extern "C" int clw_device_primes(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_
   }
   *count = n;
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// clw_device_primes without allocation, for callers that may not allocate
+// device memory. Its scratch (the count pass's counts per sieve block, their
+// offsets, the device flags and the prime list) is one file-scope device
+// array sized for the widest range the reference accepts (10^7 integers,
+// cl_wilson.cpp:1162-1166): at most 164 sieve blocks and pi(10^7 + 7) - 3 =
+// 664,576 primes. Each device holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives
+// for the current device (clwi_noalloc_setup, from clw_init).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocWidth = 10000000;
+constexpr uint32_t kNoAllocBlocks = 256;       // > 10^7 / CLW_BLOCK_SPAN + 2
+constexpr uint32_t kNoAllocPrimes = 1u << 20;  // > 664,576
+struct NoAllocScratch {
+  uint64_t primes[kNoAllocPrimes];
+  uint64_t offs[kNoAllocBlocks];
+  uint32_t counts[kNoAllocBlocks];
+  uint32_t flags[4];
+};
+static __device__ __align__(256) NoAllocScratch scratch;
+
+#define NA_CK(x)                   \
+  do {                             \
+    const int rc_ = (x);           \
+    if (rc_ != CLW_OK) return rc_; \
+  } while (0)
+
+static NoAllocScratch* g_noalloc[64];  // per device index: scratch's device address
+static std::mutex g_noalloc_mu;
+
+// With `device` current: looks up this device's copy of the scratch array once.
+int clwi_noalloc_setup(int device) {
+  if (device < 0 || device >= 64) return clw_fail(CLW_ERR_ARG, "clwi_noalloc_setup: device %d", device);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc[device]) return CLW_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return clw_fail_hip(e, "hipGetSymbolAddress(scratch)");
+  g_noalloc[device] = static_cast<NoAllocScratch*>(ptr);
+  return CLW_OK;
+}
+
+extern "C" int clw_device_primes_noalloc(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap,
+                                         uint64_t* count) {
+  if (!ctx || !count || (cap && !out)) return clw_fail(CLW_ERR_ARG, "clw_device_primes_noalloc: bad argument");
+  *count = 0;
+  if (hi <= lo) return CLW_OK;
+  if (hi - lo > kNoAllocWidth)
+    return clw_fail(CLW_ERR_ARG, "clw_device_primes_noalloc: range width %llu exceeds its scratch limit of %llu",
+                    (unsigned long long)(hi - lo), (unsigned long long)kNoAllocWidth);
+  NoAllocScratch* const S = g_noalloc[ctx->device];
+  if (!S) return clw_fail(CLW_ERR_STATE, "clw_device_primes_noalloc: scratch of device %d not set up", ctx->device);
+  uint64_t n = 0;
+  for (uint64_t p : {2ull, 3ull, 5ull})
+    if (p >= lo && p < hi) {
+      if (n < cap) out[n] = p;
+      ++n;
+    }
+  if (lo < 7) lo = 7;
+  if (hi <= lo) {
+    *count = n;
+    return CLW_OK;
+  }
+  const uint64_t base = lo / 30 * 30;
+  const uint32_t nb = (uint32_t)((hi - base + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN);
+  const uint32_t nbases = (uint32_t)mr_bases_needed(hi);
+  uint32_t hc[kNoAllocBlocks];  // host side on the stack: no allocation here either
+  uint64_t ho[kNoAllocBlocks];
+  NA_CK(clw_hip(hipMemsetAsync(S->flags, 0, sizeof(S->flags), clw_stream(ctx)), "clw_device_primes_noalloc: memset"));
+  hipLaunchKernelGGL(k_sieve_list, dim3(nb), dim3(256), 0, clw_stream(ctx), base, lo, hi, nbases, S->counts,
+                     (const uint64_t*)nullptr, (uint64_t*)nullptr, S->flags);
+  NA_CK(clw_check_launch("sieve_list"));
+  NA_CK(clw_d2h(ctx, hc, S->counts, (size_t)nb * 4, 1));
+  uint64_t tot = 0;
+  for (uint32_t k = 0; k < nb; ++k) {
+    ho[k] = tot;
+    tot += hc[k];
+  }
+  if (tot > kNoAllocPrimes)
+    return clw_fail(CLW_ERR_OVERFLOW, "clw_device_primes_noalloc: %llu primes exceed the scratch",
+                    (unsigned long long)tot);
+  NA_CK(clw_h2d(ctx, S->offs, ho, (size_t)nb * 8, 1));
+  hipLaunchKernelGGL(k_sieve_list, dim3(nb), dim3(256), 0, clw_stream(ctx), base, lo, hi, nbases, S->counts,
+                     (const uint64_t*)S->offs, S->primes, S->flags);
+  NA_CK(clw_check_launch("sieve_list"));
+  if (n < cap) NA_CK(clw_d2h(ctx, out + n, S->primes, (size_t)std::min<uint64_t>(tot, cap - n) * 8, 1));
+  uint32_t hflags = 0;
+  NA_CK(clw_d2h(ctx, &hflags, S->flags, 4, 1));
+  if (hflags) return clw_fail(CLW_ERR_OVERFLOW, "clw_device_primes_noalloc: sieve capacity exceeded");
+  *count = n + tot;
+  return CLW_OK;
 }
 
 #define DISPATCH_LK(L, KP, CALL)                                                                          \

@@ -78,6 +78,14 @@ extern "C" int clw_init(int device_index, clw_ctx** out) {
       g_tables_done[device_index] = true;
     }
   }
+  {
+    const int rc = clwi_noalloc_setup(device_index);  // with the device current
+    if (rc != CLW_OK) {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return rc;
+    }
+  }
   *out = c;
   return CLW_OK;
 }

@@ -479,6 +479,15 @@ uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap);
  * counted, not written). Replaces primesieve_generate_primes for the test
  * primes (cl_wilson.cpp:1529). */
 int clw_device_primes(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap, uint64_t* count);
+/* No counterpart in the reference: clw_device_primes for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). The same list, with no hipMalloc/hipFree and no host heap use:
+ * the count pass, offsets, flags and prime list live in one static device
+ * array per GPU (8.4 MB, held until the process ends, looked up by clw_init).
+ * Ranges are limited to hi - lo <= 10^7 (the reference's range limit,
+ * cl_wilson.cpp:1162-1166); wider ones fail with CLW_ERR_ARG. Calls that
+ * overlap on one GPU share that array and race: serialize them. */
+int clw_device_primes_noalloc(clw_ctx* ctx, uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap, uint64_t* count);
 
 #ifdef __cplusplus
 }
