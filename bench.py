@@ -57,9 +57,17 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # WRITE_SIZE; KiB per dispatch). The counters are scaled by what they read
 # for a copy of 256 MiB in the same run: FETCH_SIZE reads 131,088 KiB for the
 # 262,144 KiB read (x2.0), WRITE_SIZE 262,144 KiB (x1.0). Summary in
-# profiles/mfma_runs_cfg3_counters.md.
+# profiles/mfma_runs_cfg3_counters.md. `source_sha256` is the hash of the
+# kernel's source file when the counters were taken: if the file has changed
+# since, the figure belongs to another kernel and the line reports null.
+MFMA_SOURCE = os.path.join(ROOT, "clwilson_amd", "csrc", "clw_mfma_kernels.inc")
 MFMA_TRAFFIC = dict(fetch_kib=32013797.0, write_kib=4171220.0, fetch_scale=2.0, write_scale=1.0,
+                    source_sha256=None,
                     window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
+# The reference formulation's declared cost per sieving-prime factor
+# application at p ~ 2e13 in 32-bit partial products (mul_r2 22 + square 22.9
+# + mul 10.7 + accumulate 26): the same constant for every path of this band.
+DECLARED_MAD_PER_FACTOR = 82
 SEG_APPS_256 = 10**14  # factor applications per segment of the 256-range batch
 INT8_PEAK_OPS = 5.0e15  # MI355X dense int8 matrix-core peak (ops/s; a multiply-add is 2)
 
@@ -219,6 +227,51 @@ def measure_plain(dev, cw, regs, nxt, steps):
                 structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
 
 
+def measure_session(dev, cw, regs, nxt, steps):
+    """The session path (the reference's kernel structure: one range, one
+    launch per type and segment, lane per test prime with run products) on
+    one window per region, for the batched/session ratio."""
+    run = SessionRunner(cw, dev, 0, 0)
+    rates = []
+    try:
+        for ri in range(min(steps, 3)):
+            hi = nxt[ri]
+            lo = hi - SPAN
+            nxt[ri] = lo
+            before = run.stats()
+            t0 = time.perf_counter()
+            run.run(lo, hi)
+            cw.lib().clw_finish(dev.ctx)
+            dt = time.perf_counter() - t0
+            apps = run.stats()["mul_prime_apps"] - before["mul_prime_apps"]
+            rates.append(apps / dt)
+            say(f"session region {regs[ri][0]} {dt * 1e3:.1f} ms {apps / dt:.4g} factor apps/s")
+        t0 = time.perf_counter()
+        run.tail()
+        t_tail = time.perf_counter() - t0
+    finally:
+        run.close()
+    total = sum(F / rates[k] for k, (_, _, _, _, F) in enumerate(regs[:len(rates)]))
+    if len(rates) < 3:
+        mean = sum(rates) / len(rates)
+        total += sum(F / mean for (_, _, _, _, F) in regs[len(rates):])
+    total += t_tail
+    return dict(value=sum(run.cnt) / total, region_rates_factor_apps_per_s=rates, kernel=run.kernel,
+                projected_full_search_s=total, structure=run.name)
+
+
+def mfma_traffic_bytes():
+    """Copy-calibrated HBM bytes per k_mfma_runs launch, or None when the
+    kernel source differs from the one the counters were taken on."""
+    import hashlib
+    with open(MFMA_SOURCE, "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()
+    if not MFMA_TRAFFIC["fetch_kib"] or MFMA_TRAFFIC["source_sha256"] != sha:
+        return None
+    return (MFMA_TRAFFIC["fetch_kib"] * MFMA_TRAFFIC["fetch_scale"]
+            + MFMA_TRAFFIC["write_kib"] * MFMA_TRAFFIC["write_scale"]) * 1024
+
+
 def measure_batch256(dev, cw, steps: int, tune: bool, target_apps: float = 5e13) -> dict:
     """The 256-range batch: one batch (graph-replayed segment steps), windows
     of the sieving-prime axis at the midpoints of `steps` equal quantiles of
@@ -362,6 +415,8 @@ def main() -> int:
     ap.add_argument("--per-lane", type=int, default=0, help="test primes per lane (0 = library default)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="windows timed on the one-prime-per-launch path for the batched/plain ratio")
+    ap.add_argument("--session-steps", type=int, default=3,
+                    help="windows timed on the session path for the batched/session ratio (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=4,
                     help="windows timed on the 256-range batch (0 = skip it)")
     ap.add_argument("--no-tune", action="store_true", help="256-range batch without the autotuner")
@@ -442,6 +497,8 @@ def main() -> int:
     t_tail = time.perf_counter() - t0
     run.close()
     plain = measure_plain(dev, cw, regs, nxt, args.plain_steps) if args.plain_steps > 0 else None
+    session = (measure_session(dev, cw, regs, nxt, args.session_steps)
+               if args.session_steps > 0 and args.engine == "batch" else None)
 
     rates = [region_apps[r] / region_time[r] if region_time[r] > 0 else None for r in range(3)]
     measured = [r for r in rates if r]
@@ -474,10 +531,8 @@ def main() -> int:
         mfma_launches = max(1, ks["sieve_launches"])
         ops = 2.0 * ks["mfma_int8_macs"] / (ks["mfma_ms"] * 1e-3) if ks["mfma_ms"] else 0.0
         traffic = None
-        if (MFMA_TRAFFIC["fetch_kib"] and args.range is None and not args.tune_single
-                and run.kernel == "k_mfma_runs"):
-            traffic = (MFMA_TRAFFIC["fetch_kib"] * MFMA_TRAFFIC["fetch_scale"]
-                       + MFMA_TRAFFIC["write_kib"] * MFMA_TRAFFIC["write_scale"]) * 1024
+        if args.range is None and not args.tune_single and run.kernel == "k_mfma_runs":
+            traffic = mfma_traffic_bytes()
         roofline = {
             "bound": "mfma",
             "kernel": "k_mfma_runs",
@@ -493,10 +548,27 @@ def main() -> int:
             "int8_macs_issued": ks["mfma_int8_issued"],
             "launches": mfma_launches,
             "mean_launch_ms": ks["mfma_ms"] / mfma_launches,
+            "epilogue_partial_products": ks["mfma_partial_products"],
+            "note": "the run products' reduction mod p^2 is linear in the product's bytes, so it runs as an exact "
+                    "int8 x int8 -> int32 GEMM on the matrix cores (no floating point, no rounding); the "
+                    "vector-unit MAD roofline of the reference formulation is in declared_mad",
             "vector_units": vec_line,
         }
     else:
         roofline = vec_line
+    # the reference formulation's MAD count per factor application, the same
+    # constant for every path: a rate above the VALU peak measures how much
+    # arithmetic the run products and the matrix cores save, not a clock
+    F_steps = sum(region_apps)
+    t_steps = sum(region_time)
+    declared = dict(mad_per_factor_application=DECLARED_MAD_PER_FACTOR,
+                    factor_apps_per_s=F_steps / t_steps if t_steps else None,
+                    achieved=(F_steps / t_steps * DECLARED_MAD_PER_FACTOR / 1e12) if t_steps else None,
+                    peak=peak / 1e12, unit="Tmad/s",
+                    frac=(F_steps / t_steps * DECLARED_MAD_PER_FACTOR / peak) if t_steps and peak else None,
+                    note="above 1: the reference formulation would need this many 32-bit MADs per second on the "
+                         "vector units; the batched engine does ~1/30 of that arithmetic (run products of 128 "
+                         "sieving primes reduced once per test prime, the reduction on the matrix cores)")
     batch256 = measure_batch256(dev, cw, args.batch_steps, not args.no_tune) if args.batch_steps > 0 else None
     serving = measure_serving(args.serve_seconds) if args.serve_seconds > 0 else None
     out = {
@@ -510,7 +582,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "int (exact: 30-bit limbs in u64 column sums on the vector units, i8 x i8 -> i32 on the "
+                 "matrix cores)",
         "data": f"exact: every prime in [{PMIN}, {PMAX}); windows of the sieving-prime axis, extrapolated",
         "config": {
             "workload": ("every prime p in [2e13, 2e13+1e7) (production magnitude, p^2 ~ 2^89)"
@@ -527,9 +600,12 @@ def main() -> int:
             "autotune": getattr(run, "tune", None),
         },
         "roofline": roofline,
+        "declared_mad": declared,
         "kernel_ms": {k: ks[k] for k in ("sieve_ms", "mfma_ms", "mul_ms", "reduce_ms")},
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
+        "session": session,
+        "batched_over_session": (value / session["value"]) if session else None,
         "batch256": batch256,
         "serving": serving,
         "wall_s": wall,

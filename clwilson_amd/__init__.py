@@ -58,7 +58,7 @@ class TpTable(C.Structure):
 class SegBuffers(C.Structure):
     _fields_ = [("rec", C.c_void_p), ("nrec", C.c_void_p), ("nprimes", C.c_void_p),
                 ("side", C.c_void_p * 3), ("side_count", C.c_void_p), ("flags", C.c_void_p),
-                ("total", C.c_void_p), ("max_blocks", C.c_uint32), ("side_cap", C.c_uint32)]
+                ("total", C.c_void_p), ("era", C.c_void_p), ("max_blocks", C.c_uint32), ("side_cap", C.c_uint32)]
 
 
 class KernelStats(C.Structure):
@@ -67,7 +67,8 @@ class KernelStats(C.Structure):
                 ("side_launches", C.c_uint64), ("reduce_launches", C.c_uint64), ("sieve_ms", C.c_double),
                 ("mul_ms", C.c_double), ("side_ms", C.c_double), ("reduce_ms", C.c_double),
                 ("mul_partial_products", C.c_uint64), ("mfma_int8_macs", C.c_uint64),
-                ("mfma_ms", C.c_double), ("mfma_int8_issued", C.c_uint64)]
+                ("mfma_ms", C.c_double), ("mfma_int8_issued", C.c_uint64),
+                ("mfma_partial_products", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -156,6 +156,7 @@ struct clw_batch {
   void* subq = nullptr;         // per sub-chunk: first and last prime (uint4)
   uint32_t* subm = nullptr;     // per sub-chunk: primes | bytes << 16
   uint32_t* rowlist = nullptr;  // the segment's sub-chunks, dense (k_mfma_rows)
+  uint32_t* era = nullptr;      // wheel bitmap of the sieve of Eratosthenes (segments below 2^50)
   uint32_t* mrows = nullptr;
   bool any_mfma = false;
   SegHost* seg_d = nullptr;
@@ -197,7 +198,7 @@ struct clw_batch {
         clw_free(ctx, b.part_hi);
       }
     }
-    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, mrows};
+    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, mrows, era};
     for (void* x : ptrs) clw_free(ctx, x);
   }
 };
@@ -308,6 +309,13 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     return clw_alloc(ctx, sz, ptr);
   };
   int rc = balloc((size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
+  // every segment ends at most at maxT + 1: below 2^50 the segments are
+  // sieved by Eratosthenes (the table of sieving primes built here, before
+  // any graph capture), else by trial division and Miller-Rabin
+  if (rc == CLW_OK && B->maxT + 1 <= clwi_era_hi_max()) {
+    rc = balloc((size_t)B->max_blocks * (CLW_BLOCK_SPAN / 30), (void**)&B->era);
+    if (rc == CLW_OK) rc = clwi_era_prepare(ctx, B->maxT + 1);
+  }
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * 4, (void**)&B->nrec);
   if (rc == CLW_OK) rc = balloc(16, (void**)&B->flags);
   if (rc == CLW_OK) rc = balloc(kTotalsBytes, (void**)&B->totals);
@@ -585,7 +593,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   clw_ctx* ctx = B->ctx;
   CK(mark(B, timed));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
-  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals, B->era, B->maxT + 1));
   CK(clwi_batch_subprod(ctx, B->seg_d, extra_block, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
                         B->any_mfma ? B->subb : nullptr, B->subq, B->subm, B->flags));
   if (B->any_mfma) CK(clwi_mfma_rows(ctx, B->seg_d, extra_block, B->nrec, B->rowlist, B->mrows));
@@ -733,7 +741,7 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   clw_ctx* ctx = B->ctx;
   const uint64_t qlo = B->pending.qlo, qhi = B->pending.qhi;
   uint32_t flags[4] = {0, 0, 0, 0};
-  uint64_t tot[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint64_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   CK(clw_d2h(ctx, flags, B->flags, 16, 0));
   CK(clw_d2h(ctx, tot, B->totals, sizeof(tot), 1));
   if (flags[0]) return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded: flags %u", flags[0]);
@@ -745,6 +753,7 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   B->ks.mul_partial_products += tot[4];
   B->ks.mfma_int8_macs += tot[5];
   B->ks.mfma_int8_issued += tot[6];
+  B->ks.mfma_partial_products += tot[7];
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   for (uint32_t k = 0; k + kMarks <= B->ev_used; k += kMarks) {
     float a = 0, m = 0, b = 0, c = 0;
@@ -1020,7 +1029,7 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   CK(clw_h2d(ctx, B->seg_d, &h, sizeof(h), 1));
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
-  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals));
+  CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals, B->era, B->maxT + 1));
   CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds, nullptr,
                         nullptr, nullptr, B->flags));
   hipEvent_t e0 = nullptr, e1 = nullptr;

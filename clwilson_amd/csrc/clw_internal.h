@@ -54,7 +54,9 @@ constexpr size_t kMfmaTileBytes = (size_t)kMfmaCols * kMfmaKB * kMfmaTile;  // 1
 // Batched-engine launchers (clw_kernels.hip) used by clw_batch.hip.
 extern "C" {
 int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uint32_t max_blocks, uint32_t* flags,
-                     uint64_t* total);
+                     uint64_t* total, uint32_t* era, uint64_t era_hi_max);
+int clwi_era_prepare(clw_ctx* ctx, uint64_t hi_max);
+uint64_t clwi_era_hi_max();
 int clwi_batch_resident(clw_ctx* ctx, uint32_t limbs, uint32_t per_lane, uint32_t* out);
 int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const void* groups,
                    const uint32_t* item_gt, const void* plan, const uint32_t* rec, const uint32_t* nrec, void* seg,

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -23,6 +24,12 @@
 #include "clw_nt64.h"
 
 using namespace clw;
+
+#define NA_CK(x)                   \
+  do {                             \
+    const int rc_ = (x);           \
+    if (rc_ != CLW_OK) return rc_; \
+  } while (0)
 
 // ---------------------------------------------------------------------------
 // Sieve constants: odd primes 7..1021 and, for each prime q and each of the
@@ -109,9 +116,14 @@ struct SieveLds {
 };
 
 // The exact primes of [blo, blo + SPAN) clipped to [lo, hi), as increasing
-// offsets from blo in S.prm[0..return). Called by all 256 threads.
+// offsets from blo in S.prm[0..return). Called by all 256 threads. With
+// `era` (this block's wheel bitmap after the segment's sieve of
+// Eratosthenes by every prime up to sqrt(hi), k_era_window + k_era_large)
+// the survivors are the primes; without it the block strikes the primes
+// 7..1021 itself and tests the survivors (base-2 strong test, then the
+// deterministic Miller-Rabin bases).
 __device__ uint32_t sieve_block(SieveLds& S, uint64_t blo, uint64_t lo, uint64_t hi, uint32_t nbases,
-                                uint32_t* __restrict__ flags) {
+                                uint32_t* __restrict__ flags, const uint32_t* __restrict__ era) {
   uint32_t* const bm = S.bm;
   uint16_t* const cand = S.cand;
   uint16_t* const prm = S.prm;
@@ -123,13 +135,18 @@ __device__ uint32_t sieve_block(SieveLds& S, uint64_t blo, uint64_t lo, uint64_t
   const uint64_t vlo = blo > lo ? blo : lo;
   const uint64_t vhi = bhi < hi ? bhi : hi;
 
-  for (uint32_t i = tid; i < TURNS / 4; i += 256) bm[i] = 0xFFFFFFFFu;
-  for (uint32_t i = tid; i < CAND_CAP / 32; i += 256) pass[i] = 0;
-  if (tid == 0) s_fail = 0;
-  __syncthreads();
+  if (era) {
+    for (uint32_t i = tid; i < TURNS / 4; i += 256) bm[i] = era[i];
+    __syncthreads();
+  } else {
+    for (uint32_t i = tid; i < TURNS / 4; i += 256) bm[i] = 0xFFFFFFFFu;
+    for (uint32_t i = tid; i < CAND_CAP / 32; i += 256) pass[i] = 0;
+    if (tid == 0) s_fail = 0;
+    __syncthreads();
+  }
 
   // Trial division by 7..1021 on the wheel: strike multiples q*k, k >= q.
-  for (uint32_t pr = tid; pr < CLW_SP_COUNT * 8; pr += 256) {
+  for (uint32_t pr = tid; !era && pr < CLW_SP_COUNT * 8; pr += 256) {
     const uint32_t qi = pr >> 3, ri = pr & 7;
     const uint64_t q = c_sp[qi];
     const uint64_t start = blo > q * q ? blo : q * q;
@@ -156,6 +173,20 @@ __device__ uint32_t sieve_block(SieveLds& S, uint64_t blo, uint64_t lo, uint64_t
   }
   uint32_t ncand;
   uint32_t pos = block_exscan((uint32_t)__popcll(bits), wsum, &ncand);
+  if (era) {  // every survivor is a prime: straight into prm
+    while (bits) {
+      const int j = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      if (pos < PRIME_CAP) prm[pos] = (uint16_t)(30u * (8 * tid + (j >> 3)) + c_wheel[j & 7]);
+      ++pos;
+    }
+    if (ncand > PRIME_CAP) {
+      if (tid == 0) atomicOr(flags, CLW_FLAG_SIEVE_OVERFLOW);
+      ncand = PRIME_CAP;
+    }
+    __syncthreads();
+    return ncand;
+  }
   while (bits) {
     const int j = __ffsll((unsigned long long)bits) - 1;
     bits &= bits - 1;
@@ -234,7 +265,7 @@ __global__ __launch_bounds__(256) void k_sieve_list(uint64_t base, uint64_t lo, 
   __shared__ SieveLds S;
   const uint32_t b = blockIdx.x;
   const uint64_t blo = base + (uint64_t)b * CLW_BLOCK_SPAN;
-  const uint32_t np = sieve_block(S, blo, lo, hi, nbases, flags);
+  const uint32_t np = sieve_block(S, blo, lo, hi, nbases, flags, nullptr);
   if (!out) {
     if (threadIdx.x == 0) counts[b] = np;
     return;
@@ -250,11 +281,12 @@ __global__ __launch_bounds__(256) void k_sieve_pack(uint64_t base, uint64_t lo, 
                                                     uint32_t* __restrict__ nprimes, uint64_t* __restrict__ side0,
                                                     uint64_t* __restrict__ side1, uint64_t* __restrict__ side2,
                                                     uint32_t* __restrict__ side_count, uint32_t* __restrict__ flags,
-                                                    unsigned long long* __restrict__ total) {
+                                                    unsigned long long* __restrict__ total,
+                                                    const uint32_t* __restrict__ era) {
   __shared__ SieveLds S;
   const uint32_t tid = threadIdx.x, b = blockIdx.x;
   const uint64_t blo = base + (uint64_t)b * CLW_BLOCK_SPAN;
-  const uint32_t np = sieve_block(S, blo, lo, hi, nbases, flags);
+  const uint32_t np = sieve_block(S, blo, lo, hi, nbases, flags, era ? era + (size_t)b * (TURNS / 4) : nullptr);
   const uint16_t* const prm = S.prm;
 
   // Pair consecutive primes into records {Q limbs[5], e0, e1, e2}.
@@ -833,6 +865,7 @@ __global__ __launch_bounds__(256) void k_finalize(CTX_PARAMS, const uint64_t* __
 }
 
 #include "clw_batch_kernels.inc"
+#include "clw_era_kernels.inc"
 #include "clw_mfma_kernels.inc"
 
 // ---------------------------------------------------------------------------
@@ -876,9 +909,18 @@ extern "C" int clw_sieve_segment(clw_ctx* ctx, const clw_seg_buffers* s, uint64_
   if (nb > s->max_blocks) return clw_fail(CLW_ERR_ARG, "segment needs %llu blocks > %u", (unsigned long long)nb,
                                           s->max_blocks);
   const int nbases = mr_bases_needed(hi);
+  // below 2^50 the segment's sieve of Eratosthenes decides primality
+  const bool era = s->era && hi <= kEraHiMax;
+  if (era) {
+    const uint32_t* primes = nullptr;
+    uint32_t n_small = 0, n_large = 0;
+    NA_CK(clwi_era_table(ctx, hi, &primes, &n_small, &n_large));
+    NA_CK(era_launch(ctx, nullptr, base, hi, (uint32_t)nb, (uint32_t)nb, primes, n_small, n_large, s->era));
+  }
   hipLaunchKernelGGL(k_sieve_pack, dim3((uint32_t)nb), dim3(256), 0, clw_stream(ctx), base, lo, hi, targets[0],
                      targets[1], targets[2], (uint32_t)nbases, s->side_cap, s->rec, s->nrec, s->nprimes, s->side[0],
-                     s->side[1], s->side[2], s->side_count, s->flags, (unsigned long long*)s->total);
+                     s->side[1], s->side[2], s->side_count, s->flags, (unsigned long long*)s->total,
+                     era ? (const uint32_t*)s->era : nullptr);
   if (nblocks) *nblocks = (uint32_t)nb;
   return clw_check_launch("sieve_pack");
 }
@@ -963,12 +1005,6 @@ struct NoAllocScratch {
   uint32_t flags[4];
 };
 static __device__ __align__(256) NoAllocScratch scratch;
-
-#define NA_CK(x)                   \
-  do {                             \
-    const int rc_ = (x);           \
-    if (rc_ != CLW_OK) return rc_; \
-  } while (0)
 
 static NoAllocScratch* g_noalloc[64];  // per device index: scratch's device address
 static std::mutex g_noalloc_mu;
@@ -1157,12 +1193,31 @@ extern "C" int clw_plain_reduce(clw_ctx* ctx, const clw_tp_table* t, uint32_t tp
 }
 
 // ---- batched engine launchers (internal: called by clw_batch.hip) ----------
+// era: the segment bitmap (max_blocks sieve blocks) for the sieve of
+// Eratosthenes, used when the batch's segments end at most at era_hi_max
+// (<= 2^50; clwi_era_prepare first, outside any graph capture), else nullptr.
 extern "C" int clwi_batch_sieve(clw_ctx* ctx, void* seg, uint32_t* rec, uint32_t* nrec, uint32_t max_blocks,
-                                uint32_t* flags, uint64_t* total) {
+                                uint32_t* flags, uint64_t* total, uint32_t* era, uint64_t era_hi_max) {
+  if (era) {
+    const uint32_t* primes = nullptr;
+    uint32_t n_small = 0, n_large = 0;
+    NA_CK(clwi_era_table(ctx, era_hi_max, &primes, &n_small, &n_large));
+    NA_CK(era_launch(ctx, seg, 0, 0, 0, max_blocks, primes, n_small, n_large, era));
+  }
   hipLaunchKernelGGL(k_sieve_pairs, dim3(max_blocks), dim3(256), 0, clw_stream(ctx), (const BatchSeg*)seg, rec, nrec,
-                     max_blocks, flags, (unsigned long long*)total);
+                     max_blocks, flags, (unsigned long long*)total, (const uint32_t*)era);
   return clw_check_launch("sieve_pairs");
 }
+
+// Builds the device's table of sieving primes for the sieve of Eratosthenes
+// (once per device; not during a graph capture).
+extern "C" int clwi_era_prepare(clw_ctx* ctx, uint64_t hi_max) {
+  const uint32_t* primes = nullptr;
+  uint32_t n_small = 0, n_large = 0;
+  return clwi_era_table(ctx, hi_max, &primes, &n_small, &n_large);
+}
+
+extern "C" uint64_t clwi_era_hi_max() { return kEraHiMax; }
 
 // The batched product kernel's instantiations: limbs 2-4 x test primes per
 // lane (the autotuner's candidates).

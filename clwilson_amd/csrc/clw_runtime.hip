@@ -274,6 +274,7 @@ extern "C" int clw_seg_alloc(clw_ctx* ctx, uint32_t max_blocks, uint32_t side_ca
   if (rc == CLW_OK) rc = clw_alloc(ctx, 16, (void**)&s->side_count);
   if (rc == CLW_OK) rc = clw_alloc(ctx, 16, (void**)&s->flags);
   if (rc == CLW_OK) rc = clw_alloc(ctx, 16, (void**)&s->total);
+  if (rc == CLW_OK) rc = clw_alloc(ctx, (size_t)max_blocks * (CLW_BLOCK_SPAN / 30), (void**)&s->era);
   if (rc != CLW_OK) {
     clw_seg_free(ctx, s);
     return rc;
@@ -283,7 +284,8 @@ extern "C" int clw_seg_alloc(clw_ctx* ctx, uint32_t max_blocks, uint32_t side_ca
 
 extern "C" int clw_seg_free(clw_ctx* ctx, clw_seg_buffers* s) {
   if (!s) return CLW_OK;
-  void* ptrs[] = {s->rec, s->nrec, s->nprimes, s->side[0], s->side[1], s->side[2], s->side_count, s->flags, s->total};
+  void* ptrs[] = {s->rec,  s->nrec,       s->nprimes, s->side[0], s->side[1],
+                  s->side[2], s->side_count, s->flags,   s->total,   s->era};
   for (void* p : ptrs) clw_free(ctx, p);
   memset(s, 0, sizeof(*s));
   return CLW_OK;

@@ -142,6 +142,10 @@ typedef struct {
   uint32_t* side_count;   /* [3]                                              */
   uint32_t* flags;        /* [1] CLW_FLAG_* bits                              */
   uint64_t* total;        /* [1] primes emitted since the last clear          */
+  uint32_t* era;          /* max_blocks * CLW_BLOCK_SPAN / 30 bytes: the wheel
+                             bitmap of the segment's sieve of Eratosthenes
+                             (segments ending at most at 2^50); NULL: every
+                             segment by trial division and Miller-Rabin      */
   uint32_t max_blocks;
   uint32_t side_cap;
 } clw_seg_buffers;
@@ -155,7 +159,10 @@ int clw_seg_clear(clw_ctx* ctx, const clw_seg_buffers* s);
  * primesieve tables below 2^32 (cl_wilson.cpp:1316-1365): exact primes q in
  * [lo, hi) (lo >= 7), paired into records with their Legendre exponents for
  * the type targets T[3] (T_t = 0: type absent). Blocks start at
- * base = floor(lo/30)*30. *nblocks receives the block count. */
+ * base = floor(lo/30)*30. *nblocks receives the block count. For hi <= 2^50
+ * (and s->era allocated) a sieve of Eratosthenes by every prime up to
+ * sqrt(hi) marks the composites; above, each block strikes the primes
+ * 7..1021 and tests the survivors by deterministic Miller-Rabin. */
 int clw_sieve_segment(clw_ctx* ctx, const clw_seg_buffers* s, uint64_t lo, uint64_t hi,
                       const uint64_t targets[3], uint32_t* nblocks);
 
@@ -311,6 +318,10 @@ typedef struct {
   double mfma_ms;           /* batches: k_mfma_runs' share of mul_ms          */
   uint64_t mfma_int8_issued; /* the multiply-adds its MFMA instructions issue
                                (rows padded to groups of 48, bytes to 64)     */
+  uint64_t mfma_partial_products; /* k_mfma_runs' own 30x30-bit partial
+                               products (one Montgomery digit and one full
+                               product per sub-chunk and test prime; not in
+                               mul_partial_products)                          */
 } clw_kernel_stats;
 int clw_session_kernel_stats(const clw_session* s, clw_kernel_stats* out);
 int clw_session_reset_stats(clw_session* s);
