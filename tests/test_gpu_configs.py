@@ -122,3 +122,48 @@ def test_config5_serving_replay(dev):
         a = dev.wilson_range(r.pmin, r.pmax)
         assert np.array_equal(c.result.p, a.p) and np.array_equal(c.result.quot, a.quot), r
         assert c.result.checksum == a.checksum and c.result.results == a.results
+
+
+# Costa-Gerbicz-Harvey Table 2 values and self-test cases 4 and 7 in config 3's
+# magnitude band (1.0e13-1.75e13): their targets are at most 4.22e12 (types 0
+# and 1), so one sieve pass serves all of them.
+CGH_BAND = [10865903332033, 11273815078217, 13561740531809, 13707091918909, 14875476519749, 16556218163369,
+            16690620863071, 16880979600449, 17475368544847, 17524177394617]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("mode", [0, 2])
+def test_cgh_band_through_matrix_cores(dev, golden, mode):
+    """Full searches through one batch whose 3-limb run products go to the
+    matrix cores (mode 0: byte tables in HBM, mode 2: built per workgroup):
+    every published w_p of the ten primes of config 3's band (CGH Table 2,
+    self-test cases 4 and 7, cl_wilson.cpp:2059-2072, 2110-2123). Mode 0 runs
+    self-test case 7 as its whole range [17524177394450, 17524177394618)
+    (targets up to 8.76e12), so its checksum, result count and pi(maxT) are
+    checked too; mode 2 runs the ten primes alone."""
+    g = golden("cgh_tables.json")
+    want = {p: w for p, w in g["table1"] + g["table2"]}
+    kat7 = golden("kat_selftest.json")[6]
+    ranges = [(p, p + 1) for p in CGH_BAND[:-1]]
+    ranges.append((kat7["pmin"], kat7["pmax"]) if mode == 0 else (CGH_BAND[-1], CGH_BAND[-1] + 1))
+    with cw.Batch(dev, ranges, run_products=mode) as b:
+        assert all(st.limbs == 3 for st in b.info)
+        M = b.maxtarget
+        step = 500_000_000_000
+        for a in range(2, M + 1, step):
+            b.run(a, min(a + step, M + 1))
+            say("mode", mode, "covered", b.covered(), "of", M + 1)
+        macs = b.kernel_stats()["mfma_int8_macs"]
+        applied = b.applied()  # every prime up to the largest target, once
+        b.iterate()
+        b.finalize()
+        res = [b.results(r) for r in range(len(ranges))]
+    assert macs > 0, "no run product reached the matrix cores"
+    if mode == 0:  # case 7 holds the batch's largest target
+        assert applied == kat7["pi_maxt"]
+    for p, r in zip(CGH_BAND, res):
+        assert r.lastp == p and r.lastw == want[p], (p, r.lastw, want[p])
+    if mode == 0:
+        r7 = res[-1]
+        assert r7.resultcount == kat7["results"] and r7.checksum_hex == kat7["checksum"]
+        assert (r7.lastp, r7.lastw) == (kat7["lastp"], kat7["lastw"])
