@@ -123,11 +123,11 @@ class BatchRunner:
     """The range as a one-range batch (clw_batch_*), steps launched eagerly so
     HIP events time each kernel."""
 
-    def __init__(self, cw, dev, per_lane, tune):
+    def __init__(self, cw, dev, per_lane, tune, mfma_slices=0):
         self.cw, self.dev = cw, dev
         # one segment of SPAN integers per window, as the session path runs it
         self.b = cw.Batch(dev, [(PMIN, PMAX)], per_lane=per_lane, segment_span=SPAN, graph=False,
-                          segment_apps=10**18)
+                          segment_apps=10**18, mfma_slices=mfma_slices)
         st = self.b.info[0]
         self.T, self.cnt, self.limbs = list(st.typeTarget), list(st.tpcnt), st.limbs
         self.tune = self.b.autotune(st.typeTarget[0]) if tune else None
@@ -387,7 +387,7 @@ def profile_run(cw, args) -> int:
     y.copy_(x)
     torch.cuda.synchronize()
     dev = cw.Device(0)
-    run = BatchRunner(cw, dev, args.per_lane, False)
+    run = BatchRunner(cw, dev, args.per_lane, False, args.mfma_slices)
     top = (run.T[0] + 1) // 30 * 30
     for k in range(2):
         run.run(top - (k + 1) * SPAN, top - k * SPAN)
@@ -413,6 +413,7 @@ def main() -> int:
                     help="batched engine (one-range batch) or the session path")
     ap.add_argument("--mode", type=int, default=0, help="session product kernel (engine session)")
     ap.add_argument("--per-lane", type=int, default=0, help="test primes per lane (0 = library default)")
+    ap.add_argument("--mfma-slices", type=int, default=0, help="row slices of k_mfma_runs (0 = library default)")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="windows timed on the one-prime-per-launch path for the batched/plain ratio")
     ap.add_argument("--session-steps", type=int, default=3,
@@ -449,7 +450,7 @@ def main() -> int:
     info = dev.info()
     t_setup = time.time()
     if args.engine == "batch":
-        run = BatchRunner(cw, dev, args.per_lane, args.tune_single)
+        run = BatchRunner(cw, dev, args.per_lane, args.tune_single, args.mfma_slices)
     else:
         run = SessionRunner(cw, dev, args.mode, args.per_lane)
     T, cnt, L = run.T, run.cnt, run.limbs

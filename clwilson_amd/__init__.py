@@ -99,7 +99,7 @@ class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
                 ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32),
                 ("segment_apps", C.c_uint64), ("host_primes", C.c_uint32), ("max_chunks", C.c_uint32),
-                ("external_scratch", C.c_uint32), ("run_products", C.c_uint32)]
+                ("external_scratch", C.c_uint32), ("run_products", C.c_uint32), ("mfma_slices", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -472,7 +472,7 @@ class Batch:
     def __init__(self, dev: "Device", ranges: list[tuple[int, int]], segment_span: int = 0,
                  target_waves: int = 0, per_lane: int = 0, graph: bool = True, segment_apps: int = 0,
                  host_primes: bool = False, max_chunks: int = 0, external_scratch: bool = False,
-                 run_products: int = 0):
+                 run_products: int = 0, mfma_slices: int = 0):
         if not ranges:
             raise ValueError("a batch needs at least one range")
         self.dev = dev
@@ -482,7 +482,7 @@ class Batch:
         hi = np.array([b for _, b in self.ranges], dtype=np.uint64)
         opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0, segment_apps,
                       1 if host_primes else 0, max_chunks, 1 if external_scratch else 0,
-                      run_products)
+                      run_products, mfma_slices)
         check(lib().clw_batch_create(dev.ctx, lo.ctypes.data, hi.ctypes.data, len(self.ranges), C.byref(opt),
                                      C.byref(self.h)), "batch_create")
         self.info = []

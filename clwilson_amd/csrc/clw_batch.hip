@@ -51,7 +51,7 @@ const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 const size_t kTotalsBytes = 64;            // device counters (8 words)
 const uint32_t LB_DIGIT = 30;              // bits per digit of a run product
 const uint64_t kMfmaTableCap = 8ull << 30;  // byte tables kept in HBM up to this size
-const uint32_t kMfmaMinRows = 1024;        // sub-chunks per slice and segment for the matrix cores
+const uint32_t kMfmaMinRows = 1024;        // sub-chunks per segment / 8 for the matrix cores (a table pays off)
 const uint32_t kMarks = 5;                 // events per timed segment: start, sieved, matrix-core
                                            // run products, products, reductions
 
@@ -87,6 +87,7 @@ struct Band {
   void* ctab = nullptr;      // byte tables in HBM (allocated by the first window that uses them)
   bool ctab_tried = false;
   bool mfma_lazy = false;   // largest N below 0.4998 R: k_mfma_runs<true>
+  uint32_t nslices = kMfmaSlices;  // row slices of k_mfma_runs (partials per test prime)
   uint64_t* pm_lo = nullptr;
   uint64_t* pm_hi = nullptr;
   uint32_t n_mt() const { return (uint32_t)mt_group.size(); }
@@ -330,8 +331,14 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);
   if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subn);
   if (B->opt.run_products > 2) {
+    const uint32_t v = B->opt.run_products;
     delete B;
-    return clw_fail(CLW_ERR_ARG, "clw_options.run_products %u: 0, 1 or 2", B->opt.run_products);
+    return clw_fail(CLW_ERR_ARG, "clw_options.run_products %u: 0, 1 or 2", v);
+  }
+  if (B->opt.mfma_slices && B->opt.mfma_slices != 8 && B->opt.mfma_slices != 16 && B->opt.mfma_slices != 32) {
+    const uint32_t v = B->opt.mfma_slices;
+    delete B;
+    return clw_fail(CLW_ERR_ARG, "clw_options.mfma_slices %u: 0, 8, 16 or 32", v);
   }
   B->any_mfma = B->bands[3].L == 3 && B->opt.run_products != 1;
   if (B->any_mfma) {
@@ -390,8 +397,10 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       std::copy(bd.mt_n.begin(), bd.mt_n.end(), words.begin() + 2 * (size_t)nm);
       rc = balloc(words.size() * 4, (void**)&bd.mt_d);
       if (rc == CLW_OK) rc = clw_h2d(ctx, bd.mt_d, words.data(), words.size() * 4, 1);
-      if (rc == CLW_OK) rc = balloc((size_t)kMfmaSlices * n * 8, (void**)&bd.pm_lo);
-      if (rc == CLW_OK) rc = balloc((size_t)kMfmaSlices * n * 8, (void**)&bd.pm_hi);
+      // row slices of k_mfma_runs (16 B of partial per slice and test prime)
+      bd.nslices = B->opt.mfma_slices ? B->opt.mfma_slices : kMfmaSlices;
+      if (rc == CLW_OK) rc = balloc((size_t)bd.nslices * n * 8, (void**)&bd.pm_lo);
+      if (rc == CLW_OK) rc = balloc((size_t)bd.nslices * n * 8, (void**)&bd.pm_hi);
     }
     if (B->opt.external_scratch) {  // chunk products in a caller's buffer (clw_batch_set_scratch)
       bd.part_cap = 0;
@@ -546,7 +555,7 @@ static int mfma_window(clw_batch* B, uint64_t seg_len, uint64_t qhi) {
     if (!bd.L || !bd.mfma) continue;
     const double rows = (double)seg_len / std::log((double)std::max<uint64_t>(qhi, 16)) / (2.0 * kBatchSubRecs);
     const bool use = rows / kMfmaSlices >= kMfmaMinRows && bd.mt_active.size() > 1;
-    bd.plan.mfma = use ? 1u : 0u;
+    bd.plan.mfma = use ? bd.nslices : 0u;
     const uint32_t nm = bd.n_mt(), na = use ? (uint32_t)bd.mt_active.size() - 1 : 0u;
     if (use && !bd.ctab_tried) {
       // the byte tables in HBM (9.2 KB per test prime) up to kMfmaTableCap,
@@ -605,7 +614,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
     CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                       bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq, B->subm,
                       B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo, bd.pm_hi,
-                      B->totals + 2, bd.mfma_lazy ? 1u : 0u));
+                      B->totals + 2, bd.mfma_lazy ? 1u : 0u, bd.nslices));
   }
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {

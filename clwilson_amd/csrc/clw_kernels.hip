@@ -1468,24 +1468,27 @@ extern "C" int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, con
   return clw_check_launch("mfma_rows");
 }
 
-// grid: kMfmaSlices workgroups per tile of the band (n_mt), in rows of at most
+// grid: nslices workgroups per tile of the band (n_mt), in rows of at most
 // kMfmaMaxItemsPerLaunch tiles; those beyond the window's *n_active tiles
-// return at once.
+// return at once. nslices: a multiple of the 8 XCDs, at most kMfmaMaxSlices.
 extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, const uint32_t* mt_group,
                               const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active,
                               const uint32_t* n_active, uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows,
                               const void* subq, const uint32_t* subm, const int8_t* subb, uint64_t subb_bytes,
-                              const void* ctab, uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps, uint32_t lazy) {
+                              const void* ctab, uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps, uint32_t lazy,
+                              uint32_t nslices) {
   if (t->limbs != 3) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3-limb band only");
   if (n_mt == 0) return CLW_OK;
-  if ((uint64_t)n_mt * kMfmaSlices > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "too many test-prime tiles");
+  if (nslices % kXcds || nslices == 0 || nslices > kMfmaMaxSlices)
+    return clw_fail(CLW_ERR_ARG, "matrix-core run products: %u slices", nslices);
+  if ((uint64_t)n_mt * nslices > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "too many test-prime tiles");
   if (subb_bytes > 0x7FFFFFFFull) return clw_fail(CLW_ERR_ARG, "run-product bytes beyond 2 GiB");
-  const dim3 grid(std::min<uint32_t>(n_mt, kMfmaMaxItemsPerLaunch) * kMfmaSlices,
-                  ceil_div(n_mt, kMfmaMaxItemsPerLaunch));
+  const dim3 grid(std::min<uint32_t>(n_mt, kMfmaMaxItemsPerLaunch) * nslices, ceil_div(n_mt, kMfmaMaxItemsPerLaunch));
   auto launch = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(kMfmaThreads), 0, clw_stream(ctx), CTX_ARGS(t), (const BatchGroup*)groups,
                        mt_group, mt_first, mt_n, mt_active, n_active, rowlist, mrows, (const uint4*)subq, subm, subb,
-                       (uint32_t)subb_bytes, (const uint4*)ctab, t->count, pm_lo, pm_hi, (unsigned long long*)apps);
+                       (uint32_t)subb_bytes, (const uint4*)ctab, t->count, nslices, pm_lo, pm_hi,
+                       (unsigned long long*)apps);
   };
   if (lazy)
     launch(k_mfma_runs<true>);

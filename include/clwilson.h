@@ -251,6 +251,10 @@ typedef struct {
                               (k_batch_mul), 2 = matrix cores with the tables
                               always built per workgroup (no table memory);
                               the same results                                */
+  uint32_t mfma_slices;    /* batches, matrix cores: row slices per segment
+                              (8, 16 or 32; 0 = 8); each slice's rows are read
+                              by one XCD, each slice adds 16 B of partial per
+                              test prime; the same results                    */
 } clw_options;
 
 typedef struct {
