@@ -61,8 +61,8 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # kernel's source file when the counters were taken: if the file has changed
 # since, the figure belongs to another kernel and the line reports null.
 MFMA_SOURCE = os.path.join(ROOT, "clwilson_amd", "csrc", "clw_mfma_kernels.inc")
-MFMA_TRAFFIC = dict(fetch_kib=32013797.0, write_kib=4171220.0, fetch_scale=2.0, write_scale=1.0,
-                    source_sha256=None,
+MFMA_TRAFFIC = dict(fetch_kib=31356550.0, write_kib=2862680.0, fetch_scale=262144 / 131088, write_scale=1.0,
+                    source_sha256="eb3a6a9a94758f37d75717a781e49973a99aad0d3da716eef85a541991b77f51",
                     window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
 # The reference formulation's declared cost per sieving-prime factor
 # application at p ~ 2e13 in 32-bit partial products (mul_r2 22 + square 22.9
@@ -76,14 +76,34 @@ def say(*a):
     print("#", *a, file=sys.stderr, flush=True)
 
 
+def quantile_q(a, b, frac):
+    """q in (a, b) with pi(q) - pi(a) = frac (pi(b) - pi(a))."""
+    pa, pb = pi_approx(a), pi_approx(b)
+    want = pa + frac * (pb - pa)
+    lo, hi = float(a), float(b)
+    for _ in range(200):
+        mid = (lo + hi) / 2
+        if pi_approx(mid) < want:
+            lo = mid
+        else:
+            hi = mid
+    return int(hi)
+
+
+# Sieving primes below this are timed in full (one window, no
+# extrapolation): there the exponent runs are short, most primes take the
+# record path, and the rate is far below the bulk's.
+Q_LOW = 10**8
+
+
 def regions(T, cnt):
-    """[(name, qlo, qhi, active types, factor applications)]"""
+    """[(name, qlo, qhi, active types, factor applications)] above Q_LOW."""
     T0, T1, T2 = T
-    bounds = [("all types", 2, T0 + 1, (0, 1, 2)), ("types 1-2", T0 + 1, T1 + 1, (1, 2)),
+    bounds = [("all types", Q_LOW, T0 + 1, (0, 1, 2)), ("types 1-2", T0 + 1, T1 + 1, (1, 2)),
               ("type 2", T1 + 1, T2 + 1, (2,))]
     out = []
     for name, a, b, types in bounds:
-        nprimes = pi_approx(b - 1) - (pi_approx(a - 1) if a > 2 else 0.0)
+        nprimes = pi_approx(b - 1) - pi_approx(a - 1)
         out.append((name, a, b, types, nprimes * sum(cnt[t] for t in types)))
     return out
 
@@ -201,16 +221,15 @@ def cpu_baseline(T, cnt, tp, seconds_budget: float) -> dict:
                 factor_apps_per_s={r[0]: r[1] for r in rates})
 
 
-def measure_plain(dev, cw, regs, nxt, steps):
+def measure_plain(dev, cw, regs, wins, steps):
     """The reference's launch structure on the same workload: one test prime
-    per launch pair (mullarge + reduce), one window per region."""
+    per launch pair (mullarge + reduce), one window per region (the batched
+    leg's first window of the region)."""
     with cw.Session(dev, PMIN, PMAX, mode=3) as s:
         ntp = s.n
         rates = []
         for ri in range(min(steps, 3)):
-            hi = nxt[ri]
-            lo = hi - SPAN
-            nxt[ri] = lo
+            lo, hi = wins[ri]
             before = s.kernel_stats()
             t0 = time.perf_counter()
             s.run(lo, hi)
@@ -227,17 +246,16 @@ def measure_plain(dev, cw, regs, nxt, steps):
                 structure="one test prime per launch pair (k_plain_mul + k_plain_reduce)")
 
 
-def measure_session(dev, cw, regs, nxt, steps):
+def measure_session(dev, cw, regs, wins, steps):
     """The session path (the reference's kernel structure: one range, one
     launch per type and segment, lane per test prime with run products) on
-    one window per region, for the batched/session ratio."""
+    one window per region (the batched leg's first), for the
+    batched/session ratio."""
     run = SessionRunner(cw, dev, 0, 0)
     rates = []
     try:
         for ri in range(min(steps, 3)):
-            hi = nxt[ri]
-            lo = hi - SPAN
-            nxt[ri] = lo
+            lo, hi = wins[ri]
             before = run.stats()
             t0 = time.perf_counter()
             run.run(lo, hi)
@@ -455,9 +473,17 @@ def main() -> int:
         run = SessionRunner(cw, dev, args.mode, args.per_lane)
     T, cnt, L = run.T, run.cnt, run.limbs
     regs = regions(T, cnt)
-    # windows just below each region's upper end (multiples of 30, so each
-    # window is exactly one segment), disjoint across steps
-    nxt = [r[2] // 30 * 30 for r in regs]
+    # Step windows: region ri (cycling) gets ceil(steps / 3) windows of one
+    # segment each, ending at the midpoints of equal quantiles of its factor
+    # mass (every active test prime applies every sieving prime of the region,
+    # so the mass below q is proportional to pi(q)); the rate varies along a
+    # region (exponent runs shorten lower down), so windows stacked at its top
+    # alone would overstate it. The plain and session legs (other objects)
+    # time the same first window of each region; the warmup uses the top.
+    per_region = max(1, -(-args.steps // 3))
+    qwin = [[quantile_q(a, b, (j + 0.5) / per_region) // 30 * 30 for j in range(per_region)]
+            for (_, a, b, _, _) in regs]
+    nxt = [r[2] // 30 * 30 for r in regs]  # warmup windows, from each region's top
 
     def window(ri):
         hi = nxt[ri]
@@ -478,7 +504,8 @@ def main() -> int:
     t_all = time.perf_counter()
     for i in range(args.steps):
         ri = i % 3
-        lo, hi = window(ri)
+        hi = max(qwin[ri][i // 3], regs[ri][1] + SPAN)
+        lo = hi - SPAN
         before = run.stats()
         t0 = time.perf_counter()
         run.run(lo, hi)
@@ -492,19 +519,28 @@ def main() -> int:
         say(f"step {i} region {regs[ri][0]} [{lo}, {hi}) {dt * 1e3:.1f} ms {apps / dt:.4g} factor apps/s")
     wall = time.perf_counter() - t_all
     ks = run.stats()
+    # the sieving primes below Q_LOW, in full (a window disjoint from the steps)
+    before = run.stats()
+    t0 = time.perf_counter()
+    run.run(2, Q_LOW)
+    lib.clw_finish(dev.ctx)
+    t_low = time.perf_counter() - t0
+    low_apps = run.stats()["mul_prime_apps"] - before["mul_prime_apps"]
+    say(f"sieving primes below {Q_LOW:.0e} in full: {t_low * 1e3:.1f} ms {low_apps / t_low:.4g} factor apps/s")
     # iterate + final check once (their share of a full search)
     t0 = time.perf_counter()
     run.tail()
     t_tail = time.perf_counter() - t0
     run.close()
-    plain = measure_plain(dev, cw, regs, nxt, args.plain_steps) if args.plain_steps > 0 else None
-    session = (measure_session(dev, cw, regs, nxt, args.session_steps)
+    first = [(max(qwin[ri][0], regs[ri][1] + SPAN) - SPAN, max(qwin[ri][0], regs[ri][1] + SPAN)) for ri in range(3)]
+    plain = measure_plain(dev, cw, regs, first, args.plain_steps) if args.plain_steps > 0 else None
+    session = (measure_session(dev, cw, regs, first, args.session_steps)
                if args.session_steps > 0 and args.engine == "batch" else None)
 
     rates = [region_apps[r] / region_time[r] if region_time[r] > 0 else None for r in range(3)]
     measured = [r for r in rates if r]
     mean_rate = sum(measured) / len(measured)
-    total = sum(F / (rates[k] or mean_rate) for k, (_, _, _, _, F) in enumerate(regs)) + t_tail
+    total = sum(F / (rates[k] or mean_rate) for k, (_, _, _, _, F) in enumerate(regs)) + t_low + t_tail
     value = sum(cnt) / total
 
     peak = info["compute_units"] * 64 * info["clock_khz"] * 1e3  # v_mad_u64_u32 / s
@@ -593,9 +629,11 @@ def main() -> int:
             "engine": run.name,
             "test_primes": sum(cnt), "types": cnt, "type_targets": T, "limbs": L,
             "step": f"one segment of {SPAN} sieving integers for all test primes",
-            "factor_applications_total": sum(r[4] for r in regs),
+            "factor_applications_total": sum(r[4] for r in regs) + low_apps,
             "region_rates_factor_apps_per_s": {regs[k][0]: rates[k] for k in range(3)},
             "iterate_finalize_s": t_tail,
+            "below_q_low_s": t_low,
+            "q_low": Q_LOW,
             "projected_full_search_s": total,
             "per_lane": run.kp,
             "autotune": getattr(run, "tune", None),
