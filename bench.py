@@ -381,7 +381,7 @@ def measure_serving(seconds: float) -> dict:
     arrival) and the factor applications/s sustained over it."""
     from clwilson_amd.serve import Server
     from clwilson_amd.workloads import serving_requests
-    srv = Server(hbm_budget=64 << 30, streams=3, step_seconds=0.05)
+    srv = Server(hbm_budget=64 << 30, streams=3)
     try:
         rep = srv.serve(serving_requests(), duration=seconds, progress=lambda r: say(
             f"serving t={r['wall_s']:.0f}s arrived {r['arrived']} completed {r['completed']} "

@@ -108,9 +108,9 @@ class Server:
     gives a process by default, so no stream waits behind another's kernel
     in a shared queue."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.05,
-                 apps_per_second: float = 2e12, segment_blocks: int = 64, max_cohort: int = 64,
-                 admit_interval: float = 0.5):
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.2,
+                 apps_per_second: float = 1e13, segment_blocks: int = 640, max_cohort: int = 64,
+                 admit_interval: float = 0.5, host_primes: bool = True):
         self.devices = [Device(device) for _ in range(streams)]
         self.admit_dev = Device(device)
         # chunk products of every cohort on a stream live in that stream's
@@ -119,7 +119,11 @@ class Server:
         self.budget = hbm_budget
         self.step_seconds = step_seconds
         self.rate = apps_per_second
+        # segments of 640 blocks (39 M integers, ~1.4 M sieving primes at 1e12):
+        # long enough for the 3-limb band's run products to go to the matrix
+        # cores (about a thousand sub-chunks per slice)
         self.segment_span = segment_blocks * BLOCK_SPAN
+        self.host_primes = host_primes
         self.max_cohort = max_cohort
         self.admit_interval = admit_interval
         self.lock = threading.Lock()
@@ -230,7 +234,8 @@ class Server:
     def _build(self, take: list[Request]) -> _Cohort:
         """A cohort of requests: their batch built on the admission stream."""
         b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
-                  segment_apps=int(self.rate * self.step_seconds), host_primes=True, external_scratch=True,
+                  segment_apps=int(self.rate * self.step_seconds), host_primes=self.host_primes,
+                  external_scratch=True,
                   run_products=2)
         groups = [(st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t]]
         c = _Cohort(b, -1, take, [st.maxtarget for st in b.info], np.array([g[0] for g in groups], dtype=np.float64),

@@ -39,7 +39,7 @@ def test_serving_alone_vs_mix(dev, streams):
     (the session path), and the budget is never exceeded."""
     reqs = [Request(i, 0.025 * i, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
     budget = streams * Server.SCRATCH_SLOTS * 16 + (30 << 20)  # per-stream scratch + ~3 cohorts
-    srv = Server(hbm_budget=budget, streams=streams, step_seconds=0.01, admit_interval=0.05)
+    srv = Server(hbm_budget=budget, streams=streams, step_seconds=0.01, admit_interval=0.05, segment_blocks=64)
     try:
         rep = srv.serve(reqs, duration=100.0, until_done=True)
     finally:
@@ -81,7 +81,7 @@ def test_server_snapshot_restore(dev, tmp_path):
     """A server stopped part-way and snapshotted, then restored into a new
     server, completes every request with the single-range results."""
     reqs = [Request(i, 0.0, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
-    srv = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.002)
+    srv = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.002, segment_blocks=64)
     try:
         for r in reqs:
             srv.submit(r)
@@ -93,7 +93,7 @@ def test_server_snapshot_restore(dev, tmp_path):
     finally:
         srv.close()
     say("completed before the snapshot:", sorted(before))
-    srv2 = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.01)
+    srv2 = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.01, segment_blocks=64)
     try:
         done = srv2.restore(str(tmp_path))
         assert sorted(done) == sorted(before)
