@@ -51,6 +51,7 @@ const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 const size_t kTotalsBytes = 64;            // device counters (8 words)
 const uint32_t LB_DIGIT = 30;              // bits per digit of a run product
 const uint64_t kMfmaTableCap = 8ull << 30;  // byte tables kept in HBM up to this size
+const uint64_t kMfmaMaxP4 = 1ull << 47;   // 4-limb band on the matrix cores below this p
 const uint32_t kMfmaMinRows = 1024;        // sub-chunks per segment / 8 for the matrix cores (a table pays off)
 const uint32_t kMarks = 5;                 // events per timed segment: start, sieved, matrix-core
                                            // run products, products, reductions
@@ -340,13 +341,24 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     delete B;
     return clw_fail(CLW_ERR_ARG, "clw_options.mfma_slices %u: 0, 8, 16 or 32", v);
   }
-  B->any_mfma = B->bands[3].L == 3 && B->opt.run_products != 1;
+  // Matrix-core run products: the 3-limb band, and the 4-limb band while its
+  // test primes stay below 2^47 (N < 2^94: the same 12-byte tables, and
+  // sub-chunk products of primes below its targets, < 2^46, within kMfmaKB
+  // bytes)
+  B->any_mfma = false;
+  for (int L = 3; L <= 4 && B->opt.run_products != 1; ++L) {
+    Band& bd = B->bands[L];
+    if (bd.L != L) continue;
+    uint64_t pmax = 0;
+    for (uint64_t p : bd.p) pmax = std::max(pmax, p);
+    if (L == 4 && pmax >= kMfmaMaxP4) continue;
+    bd.mfma = true;
+    // lazy run accumulator: 2 p^2 < 0.9996 R for every test prime of the band
+    // (R = 2^90 at 3 limbs; at 4 limbs N < 2^94 is far below R = 2^120)
+    bd.mfma_lazy = L == 4 || 2.0 * (double)pmax * (double)pmax < 0.9996 * std::ldexp(1.0, 90);
+    B->any_mfma = true;
+  }
   if (B->any_mfma) {
-    B->bands[3].mfma = true;
-    // 2 p^2 < 0.9996 R (R = 2^90) for every test prime of the band
-    uint64_t pmax3 = 0;
-    for (uint64_t p : B->bands[3].p) pmax3 = std::max(pmax3, p);
-    B->bands[3].mfma_lazy = 2.0 * (double)pmax3 * (double)pmax3 < 0.9996 * std::ldexp(1.0, 90);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * kMfmaKB, (void**)&B->subb);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 16, &B->subq);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subm);

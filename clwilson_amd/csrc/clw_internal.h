@@ -38,7 +38,8 @@ constexpr uint32_t kBatchSubRecs = 64;
 constexpr uint32_t kBatchSubCap = CLW_REC_CAP / kBatchSubRecs;
 constexpr uint32_t kBatchSubMaxDigits = 276;
 
-// Run products on the matrix cores (3-limb band): each sub-chunk product P
+// Run products on the matrix cores (the 3-limb band, and the 4-limb band while
+// its test primes stay below 2^47): each sub-chunk product P
 // is also kept as kMfmaKB signed bytes a_i (P = sum a_i 2^(8 i), a_i in
 // [-128, 127]); every test prime of the band has a table of the
 // kMfmaCols signed bytes of c_i = 2^(8 i + 30) mod p^2 for i < kMfmaKB, so that
@@ -46,8 +47,8 @@ constexpr uint32_t kBatchSubMaxDigits = 276;
 // test primes x byte columns). kMfmaTile test primes share one table tile in
 // LDS; the rows of a segment are split into kMfmaSlices slices.
 constexpr uint32_t kMfmaTile = 16;
-constexpr uint32_t kMfmaCols = 12;   // bytes of c < p^2 < 2^89
-constexpr uint32_t kMfmaKB = 768;    // bytes per product: 128 primes below 2^44 take <= 705
+constexpr uint32_t kMfmaCols = 12;   // bytes of c < p^2 < 2^94
+constexpr uint32_t kMfmaKB = 768;    // bytes per product: 128 primes below 2^46 take <= 737
 constexpr uint32_t kMfmaSlices = 8;      // fewest slices (one per XCD)
 constexpr uint32_t kMfmaMaxSlices = 32;  // most: a slice's rows then stay in one XCD's L2
 constexpr size_t kMfmaTileBytes = (size_t)kMfmaCols * kMfmaKB * kMfmaTile;  // 147,456

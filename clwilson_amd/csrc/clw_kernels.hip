@@ -1454,7 +1454,7 @@ extern "C" int clwi_batch_reduce(clw_ctx* ctx, const clw_tp_table* t, const uint
 // primes (tile mt: band entries mt_first[mt] + [0, mt_n[mt])).
 extern "C" int clwi_mfma_table(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* mt_first, const uint32_t* mt_n,
                                uint32_t n_mt, void* ctab) {
-  if (t->limbs != 3) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3-limb band only");
+  if (t->limbs != 3 && t->limbs != 4) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3 or 4 limbs");
   if (n_mt == 0) return CLW_OK;
   hipLaunchKernelGGL(k_mfma_table, dim3(ceil_div(n_mt * kMfmaTile, 256)), dim3(256), 0, clw_stream(ctx), t->n_lo,
                      t->n_hi, mt_first, mt_n, n_mt, (uint4*)ctab);
@@ -1477,7 +1477,7 @@ extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* g
                               const void* subq, const uint32_t* subm, const int8_t* subb, uint64_t subb_bytes,
                               const void* ctab, uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps, uint32_t lazy,
                               uint32_t nslices) {
-  if (t->limbs != 3) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3-limb band only");
+  if (t->limbs != 3 && t->limbs != 4) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3 or 4 limbs");
   if (n_mt == 0) return CLW_OK;
   if (nslices % kXcds || nslices == 0 || nslices > kMfmaMaxSlices)
     return clw_fail(CLW_ERR_ARG, "matrix-core run products: %u slices", nslices);
@@ -1490,10 +1490,12 @@ extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* g
                        (uint32_t)subb_bytes, (const uint4*)ctab, t->count, nslices, pm_lo, pm_hi,
                        (unsigned long long*)apps);
   };
-  if (lazy)
-    launch(k_mfma_runs<true>);
+  if (t->limbs == 4)  // N < 2^94 (the batch takes p < 2^47 here): the lazy chain always holds
+    launch(k_mfma_runs<4, true>);
+  else if (lazy)
+    launch(k_mfma_runs<3, true>);
   else
-    launch(k_mfma_runs<false>);
+    launch(k_mfma_runs<3, false>);
   return clw_check_launch("mfma_runs");
 }
 

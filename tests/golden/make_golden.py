@@ -142,6 +142,10 @@ def type_targets(ps) -> tuple[list[int], dict[int, int]]:
 MFMA_CASES = [  # (pmin, pmax, [window centre: an exponent-run boundary or a target])
     (20_000_000_000_000, 20_000_000_001_200, ["T2/2", "T2/3", "T1/2", "T0-"]),
     (24_876_000_000_000, 24_876_000_001_600, ["T2/2", "T0-"]),
+    # 4-limb band (p above 2.49e13) on the matrix cores, up to its limit 2^47
+    (30_000_000_000_000, 30_000_000_001_600, ["T2/2", "T0-"]),
+    (100_000_000_000_000, 100_000_000_001_800, ["T2/3", "T1/2"]),
+    (140_737_000_000_000, 140_737_000_001_800, ["T2/2", "T0-"]),
 ]
 MFMA_HALF = 20_000_000  # windows of 4e7 integers (~1.4 M primes, ~10,800 sub-chunks)
 

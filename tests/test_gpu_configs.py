@@ -1,8 +1,9 @@
 """GPU parity on the benchmark workloads and on the matrix-core run products.
 
 * the matrix cores (k_mfma_runs, byte tables from HBM and built per
-  workgroup) against the oracle's window products at p ~ 2e13 and at the top
-  of the 3-limb band, on windows long enough for the engine to pick them;
+  workgroup) against the oracle's window products in the 3-limb band (p ~ 2e13
+  and its top) and the 4-limb band (p ~ 3e13, 1e14, just below 2^47), on
+  windows long enough for the engine to pick them;
 * every prime in [1e11, 1e11+1e7) (config 2): the full search through the
   batched engine, sampled primes against the oracle's full plain method;
 * the 256-range batch (config 4): two windows of sieving primes, sampled
@@ -29,20 +30,41 @@ def say(*a):
 
 @pytest.mark.parametrize("mode", [0, 2])
 def test_matrix_core_windows_vs_oracle(dev, golden, mode):
-    """Window products through a batch whose 3-limb run products go to the
-    matrix cores (mode 0: byte tables in HBM; mode 2: built per workgroup in
-    LDS) equal the oracle's (mullarge.cl:36-64 restated), at p ~ 2e13 (lazy
-    run accumulator) and p ~ 2.4876e13 (conditional subtraction kept), on
-    windows across T/2 and T/3 exponent-run boundaries and type targets."""
+    """Window products through a batch whose run products go to the matrix
+    cores (mode 0: byte tables in HBM; mode 2: built per workgroup in LDS)
+    equal the oracle's (mullarge.cl:36-64 restated): 3 limbs at p ~ 2e13
+    (lazy run accumulator) and p ~ 2.4876e13 (conditional subtraction kept),
+    4 limbs at p ~ 3e13, 1e14 and just below 2^47, on windows across T/2 and
+    T/3 exponent-run boundaries and type targets."""
     for w in golden("mfma_windows.json"):
         say("window", w["pmin"], w["at"], w["qlo"], w["qhi"], "mode", mode)
         with cw.Batch(dev, [(w["pmin"], w["pmax"])], run_products=mode) as b:
-            assert b.info[0].limbs == 3 and b.info[0].tpcount >= 32
+            assert b.info[0].limbs == (3 if w["pmax"] < 24_880_000_000_000 else 4)
+            assert b.info[0].tpcount >= 32
             assert list(b.info[0].typeTarget) == [w["type_targets"][str(t)] for t in range(3)]
             b.run(w["qlo"], w["qhi"])
             ks = b.kernel_stats()
             assert ks["mfma_int8_macs"] > 0, "the window did not reach the matrix cores"
             assert b.residues(0) == [int(v) for v in w["value"]]
+
+
+def test_matrix_cores_stop_at_2_47(dev):
+    """Test primes at or above 2^47 keep the scalar run products (their
+    sub-chunk products no longer fit the byte tables): a window there runs
+    no matrix-core work and equals the same window with the matrix cores
+    switched off, while a range just below 2^47 takes them."""
+    q = (35_184_230_000_016, 35_184_270_000_016)
+    for lo, on in ((140_737_488_355_400, False), (140_737_000_000_000, True)):
+        out = []
+        for mode in (0, 1):
+            with cw.Batch(dev, [(lo, lo + 1800)], run_products=mode) as b:
+                assert b.info[0].limbs == 4
+                b.run(*q)
+                out.append(b.residues(0))
+                macs = b.kernel_stats()["mfma_int8_macs"]
+                say("p", lo, "mode", mode, "int8 MACs", macs)
+                assert (macs > 0) == (on and mode == 0)
+        assert out[0] == out[1]
 
 
 @pytest.mark.slow
