@@ -150,7 +150,8 @@ class BatchRunner:
                           segment_apps=10**18, mfma_slices=mfma_slices)
         st = self.b.info[0]
         self.T, self.cnt, self.limbs = list(st.typeTarget), list(st.tpcnt), st.limbs
-        self.tune = self.b.autotune(st.typeTarget[0]) if tune else None
+        # tuned at the top of the all-types region, where the timed windows start
+        self.tune = self.b.autotune(max(7, st.typeTarget[0] - 2 * SPAN)) if tune else None
         self.kp = self.tune[0]["per_lane"] if tune else (per_lane or int(cw.lib().clw_batch_default_per_lane(self.limbs)))
         self.vec_kernel = f"k_batch_mul<{self.limbs},{self.kp}>"
         # 3 limbs: the run products go to the matrix cores
@@ -438,10 +439,7 @@ def main() -> int:
                     help="windows timed on the session path for the batched/session ratio (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=4,
                     help="windows timed on the 256-range batch (0 = skip it)")
-    ap.add_argument("--no-tune", action="store_true", help="256-range batch without the autotuner")
-    ap.add_argument("--tune-single", action="store_true",
-                    help="autotune the one-range batch too (default: 3 test primes per lane, which the "
-                         "autotuner finds best at 3 limbs)")
+    ap.add_argument("--no-tune", action="store_true", help="batches without the autotuner")
     ap.add_argument("--serve-seconds", type=float, default=45.0,
                     help="wall-clock window of the serving replay (0 = skip it)")
     ap.add_argument("--range", type=int, nargs=2, metavar=("PMIN", "PMAX"), default=None,
@@ -468,7 +466,7 @@ def main() -> int:
     info = dev.info()
     t_setup = time.time()
     if args.engine == "batch":
-        run = BatchRunner(cw, dev, args.per_lane, args.tune_single, args.mfma_slices)
+        run = BatchRunner(cw, dev, args.per_lane, not args.no_tune, args.mfma_slices)
     else:
         run = SessionRunner(cw, dev, args.mode, args.per_lane)
     T, cnt, L = run.T, run.cnt, run.limbs
@@ -568,7 +566,10 @@ def main() -> int:
         mfma_launches = max(1, ks["sieve_launches"])
         ops = 2.0 * ks["mfma_int8_macs"] / (ks["mfma_ms"] * 1e-3) if ks["mfma_ms"] else 0.0
         traffic = None
-        if args.range is None and not args.tune_single and run.kernel == "k_mfma_runs":
+        # the counters were taken with 8 row slices (the untuned default)
+        tuned = getattr(run, "tune", None)
+        slices = tuned[0].get("mfma_slices", 8) if tuned else (args.mfma_slices or 8)
+        if args.range is None and slices == 8 and run.kernel == "k_mfma_runs":
             traffic = mfma_traffic_bytes()
         roofline = {
             "bound": "mfma",

@@ -78,15 +78,22 @@ class BandTune(C.Structure):
     _fields_ = [("limbs", C.c_uint32), ("per_lane", C.c_uint32), ("items_per_block", C.c_uint32),
                 ("trials", C.c_uint32), ("from_cache", C.c_uint32), ("active_test_primes", C.c_uint64),
                 ("best_apps_per_s", C.c_double), ("trial_per_lane", C.c_uint32 * 16),
-                ("trial_items_per_block", C.c_uint32 * 16), ("trial_apps_per_s", C.c_double * 16)]
+                ("trial_items_per_block", C.c_uint32 * 16), ("trial_apps_per_s", C.c_double * 16),
+                ("mfma_slices", C.c_uint32), ("mfma_min_rows", C.c_uint32), ("mfma_trials", C.c_uint32),
+                ("mfma_trial_slices", C.c_uint32 * 8), ("mfma_trial_rows", C.c_uint64 * 8),
+                ("mfma_trial_ms", C.c_double * 8)]
 
     def as_dict(self) -> dict:
         n = min(self.trials, 16)
+        m = min(self.mfma_trials, 8)
         return dict(limbs=self.limbs, per_lane=self.per_lane, items_per_block=self.items_per_block,
                     trials=self.trials, from_cache=bool(self.from_cache),
                     active_test_primes=self.active_test_primes, best_apps_per_s=self.best_apps_per_s,
                     candidates=[(self.trial_per_lane[k], self.trial_items_per_block[k], self.trial_apps_per_s[k])
-                                for k in range(n)])
+                                for k in range(n)],
+                    mfma_slices=self.mfma_slices, mfma_min_rows=self.mfma_min_rows,
+                    mfma_candidates=[(self.mfma_trial_slices[k], self.mfma_trial_rows[k], self.mfma_trial_ms[k])
+                                     for k in range(m)])
 
 
 class WorkStatus(C.Structure):
@@ -615,8 +622,10 @@ class Batch:
         return ks.as_dict()
 
     def autotune(self, q: int, cache_path: str | None = None) -> list[dict]:
-        """Per limb band: test primes per lane and items per resident block,
-        timed on a trial segment at magnitude q (no residue changes)."""
+        """Per limb band: the matrix-core row slices and switch (bands with
+        matrix-core run products), then test primes per lane and items per
+        resident block, timed on trial segments at magnitude q (no residue
+        changes)."""
         rep = (BandTune * 4)()
         check(lib().clw_batch_autotune(self.h, q, cache_path.encode() if cache_path else None, rep, 4),
               "batch_autotune")

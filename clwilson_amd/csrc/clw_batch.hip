@@ -89,6 +89,9 @@ struct Band {
   bool ctab_tried = false;
   bool mfma_lazy = false;   // largest N below 0.4998 R: k_mfma_runs<true>
   uint32_t nslices = kMfmaSlices;  // row slices of k_mfma_runs (partials per test prime)
+  uint32_t pm_slices = 0;          // slices pm_lo/pm_hi hold
+  uint32_t mfma_min_rows = kMfmaMinRows;  // sub-chunks per slice and segment from which the matrix cores
+                                          // take the run products (the autotuner measures it)
   uint64_t* pm_lo = nullptr;
   uint64_t* pm_hi = nullptr;
   uint32_t n_mt() const { return (uint32_t)mt_group.size(); }
@@ -414,6 +417,7 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       bd.nslices = B->opt.mfma_slices ? B->opt.mfma_slices : kMfmaSlices;
       if (rc == CLW_OK) rc = balloc((size_t)bd.nslices * n * 8, (void**)&bd.pm_lo);
       if (rc == CLW_OK) rc = balloc((size_t)bd.nslices * n * 8, (void**)&bd.pm_hi);
+      bd.pm_slices = bd.nslices;
     }
     if (B->opt.external_scratch) {  // chunk products in a caller's buffer (clw_batch_set_scratch)
       bd.part_cap = 0;
@@ -563,35 +567,48 @@ static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
 // primes, which pays off over about a thousand sub-chunks of its slice; short
 // segments (large batches at the default segment_apps) keep k_batch_mul's
 // reduction. Sets the plan flag both kernels read and the tile count.
+// The byte tables in HBM (9.2 KB per test prime) up to kMfmaTableCap, made
+// when a window (or a tuning trial) first uses the matrix cores; without them
+// (run_products 2, larger, or no memory) k_mfma_runs builds each tile's table
+// in LDS.
+static int mfma_tables(clw_batch* B, Band& bd) {
+  if (bd.ctab_tried) return CLW_OK;
+  bd.ctab_tried = true;
+  const uint32_t nm = bd.n_mt();
+  if (B->opt.run_products == 0 && (uint64_t)nm * kMfmaTileBytes <= kMfmaTableCap &&
+      hipMalloc(&bd.ctab, (size_t)nm * kMfmaTileBytes) == hipSuccess) {
+    B->bytes += (uint64_t)nm * kMfmaTileBytes;
+    CK(clwi_mfma_table(B->ctx, &bd.tab, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm, nm, bd.ctab));
+    if (B->graph_exec) {  // captured without the tables: capture the segment step again
+      CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
+      CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
+      B->graph_exec = nullptr;
+      B->graph = nullptr;
+    }
+  } else {
+    bd.ctab = nullptr;
+    (void)hipGetLastError();  // a failed allocation is not an error here
+  }
+  return CLW_OK;
+}
+
+// Sets the matrix-core flag (the slice count, or 0 for the vector units) of
+// the band's plan and the count of active tiles, both read by the kernels.
+static int mfma_plan(clw_batch* B, Band& bd, uint32_t slices) {
+  bd.plan.mfma = slices;
+  const uint32_t nm = bd.n_mt(), na = slices ? (uint32_t)bd.mt_active.size() - 1 : 0u;
+  if (slices) CK(mfma_tables(B, bd));
+  CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 0));
+  CK(clw_h2d(B->ctx, bd.mt_d + 4 * (size_t)nm, &na, 4, 1));
+  return CLW_OK;
+}
+
 static int mfma_window(clw_batch* B, uint64_t seg_len, uint64_t qhi) {
   for (Band& bd : B->bands) {
     if (!bd.L || !bd.mfma) continue;
     const double rows = (double)seg_len / std::log((double)std::max<uint64_t>(qhi, 16)) / (2.0 * kBatchSubRecs);
-    const bool use = rows / kMfmaSlices >= kMfmaMinRows && bd.mt_active.size() > 1;
-    bd.plan.mfma = use ? bd.nslices : 0u;
-    const uint32_t nm = bd.n_mt(), na = use ? (uint32_t)bd.mt_active.size() - 1 : 0u;
-    if (use && !bd.ctab_tried) {
-      // the byte tables in HBM (9.2 KB per test prime) up to kMfmaTableCap,
-      // on the first window that uses them; without them (run_products 2,
-      // larger, or no memory) k_mfma_runs builds each tile's table in LDS
-      bd.ctab_tried = true;
-      if (B->opt.run_products == 0 && (uint64_t)nm * kMfmaTileBytes <= kMfmaTableCap &&
-          hipMalloc(&bd.ctab, (size_t)nm * kMfmaTileBytes) == hipSuccess) {
-        B->bytes += (uint64_t)nm * kMfmaTileBytes;
-        CK(clwi_mfma_table(B->ctx, &bd.tab, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm, nm, bd.ctab));
-        if (B->graph_exec) {  // captured without the tables: capture the segment step again
-          CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
-          CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
-          B->graph_exec = nullptr;
-          B->graph = nullptr;
-        }
-      } else {
-        bd.ctab = nullptr;
-        (void)hipGetLastError();  // a failed allocation is not an error here
-      }
-    }
-    CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 0));
-    CK(clw_h2d(B->ctx, bd.mt_d + 4 * (size_t)nm, &na, 4, 1));
+    const bool use = rows / bd.nslices >= bd.mfma_min_rows && bd.mt_active.size() > 1;
+    CK(mfma_plan(B, bd, use ? bd.nslices : 0u));
   }
   return CLW_OK;
 }
@@ -990,14 +1007,23 @@ extern "C" int clw_batch_kernel_stats(const clw_batch* B, clw_kernel_stats* out)
 
 // ---- autotuner -------------------------------------------------------------
 //
-// Per limb band, picks the test primes per lane of the product kernel (1, 2
-// or 4 Montgomery products in lockstep, i.e. the tile of 256 x KP test
-// primes a work-group carries) and the work items per resident block (the
-// chunk count K, i.e. the fan-in of the reduction that folds chunk products).
-// Each candidate is timed on one trial segment at the caller's magnitude q:
-// sieve, then the band's product kernel only, writing chunk products to
-// scratch, so no accumulator changes. Results can be kept in a text cache
-// keyed by device, band and the log2 of the band's active test primes.
+// Per limb band, at the caller's magnitude q (the reference sizes its
+// launches from a timed probe instead, cl_wilson.cpp:1184-1290):
+//  1. for a band with matrix-core run products, the row slices of
+//     k_mfma_runs (8 or 16) and the switch between the matrix cores and the
+//     vector units: both product paths are timed on trial segments of the
+//     length a window at q runs and of a quarter of it; the matrix-core time
+//     is a + b r and the vector time c r in the rows r per slice, so the
+//     matrix cores pay from r* = a / (c - b) rows per slice on
+//     (mfma_min_rows; never if c <= b);
+//  2. the test primes per lane of k_batch_mul (1, 2 or 4 Montgomery products
+//     in lockstep, i.e. the tile of 256 x KP test primes a work-group
+//     carries) x work items per resident block (the chunk count K, i.e. the
+//     fan-in of the reduction that folds chunk products), timed with the run
+//     products where step 1 put them.
+// Trials write chunk products and partials to scratch and run no reduction,
+// so no residue changes. Results can be kept in a text cache keyed by device,
+// band and the log2 of the band's active test primes.
 
 namespace {
 
@@ -1006,18 +1032,25 @@ struct TuneKey {
   int cus, L, size_class;
 };
 
-bool cache_lookup(const char* path, const TuneKey& k, uint32_t* kp, uint32_t* ipb) {
+struct TuneChoice {
+  uint32_t kp = 0, ipb = 0, slices = 0, min_rows = 0;
+};
+
+const char kTuneTag[] = "clwtune2";  // cache lines: tag arch cus L size_class kp ipb slices min_rows
+
+bool cache_lookup(const char* path, const TuneKey& k, TuneChoice* c) {
   if (!path) return false;
   FILE* f = fopen(path, "r");
   if (!f) return false;
-  char arch[64];
-  int cus, L, sc;
-  unsigned a, b;
+  char line[256];
   bool hit = false;
-  while (fscanf(f, "%63s %d %d %d %u %u", arch, &cus, &L, &sc, &a, &b) == 6) {
-    if (!strcmp(arch, k.arch) && cus == k.cus && L == k.L && sc == k.size_class) {
-      *kp = a;
-      *ipb = b;
+  while (fgets(line, sizeof(line), f)) {
+    char tag[16], arch[64];
+    int cus, L, sc;
+    unsigned a, b, sl, mr;
+    if (sscanf(line, "%15s %63s %d %d %d %u %u %u %u", tag, arch, &cus, &L, &sc, &a, &b, &sl, &mr) != 9) continue;
+    if (!strcmp(tag, kTuneTag) && !strcmp(arch, k.arch) && cus == k.cus && L == k.L && sc == k.size_class) {
+      *c = TuneChoice{a, b, sl, mr};
       hit = true;  // the last matching line wins
     }
   }
@@ -1025,24 +1058,33 @@ bool cache_lookup(const char* path, const TuneKey& k, uint32_t* kp, uint32_t* ip
   return hit;
 }
 
-void cache_store(const char* path, const TuneKey& k, uint32_t kp, uint32_t ipb) {
+void cache_store(const char* path, const TuneKey& k, const TuneChoice& c) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %u %u\n", k.arch, k.cus, k.L, k.size_class, kp, ipb);
+  fprintf(f, "%s %s %d %d %d %u %u %u %u\n", kTuneTag, k.arch, k.cus, k.L, k.size_class, c.kp, c.ipb, c.slices,
+          c.min_rows);
   fclose(f);
 }
 
 }  // namespace
 
-// One trial: the band's product kernel on the segment [q, q + span) of the
-// records, timed with events. Returns factor applications per second.
-static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, double* rate) {
+// Segment blocks of a window at q with `active` test primes (the length
+// clw_batch_launch would choose).
+static uint64_t window_blocks(const clw_batch* B, uint64_t q, uint64_t active) {
+  const double apps = B->opt.segment_apps ? (double)B->opt.segment_apps : kAppsPerSegment;
+  const double per_block = (double)std::max<uint64_t>(active, 1) * CLW_BLOCK_SPAN /
+                           std::log((double)std::max<uint64_t>(q, 16));
+  return std::min<uint64_t>(B->max_blocks - 2, std::max<uint64_t>(kMinSpanBlocks, (uint64_t)(apps / per_block) + 1));
+}
+
+// Trial segment [q, q + span_blocks blocks) (clipped to maxT): sieved, run
+// products formed (digits, and for the matrix cores bytes and the dense row
+// list). Returns the segment's sub-chunks (rows); *desc receives the device
+// segment descriptor as the product kernels find it (work queues empty), so
+// that each trial on the segment can start from it.
+static int trial_segment(clw_batch* B, uint64_t q, uint64_t span_blocks, uint32_t* rows, SegHost* desc) {
   clw_ctx* ctx = B->ctx;
-  uint64_t active = 0;
-  CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
-  if (B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
-  CK(plan_window(B, q, &active));
   SegHost h{};
   h.lo = q;
   h.hi = q;
@@ -1052,26 +1094,132 @@ static int tune_trial(clw_batch* B, Band& bd, uint64_t q, uint64_t span_blocks, 
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals, B->era, B->maxT + 1));
-  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds, nullptr,
-                        nullptr, nullptr, B->flags));
+  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
+                        B->any_mfma ? B->subb : nullptr, B->subq, B->subm, B->flags));
+  *rows = 0;
+  if (B->any_mfma) {
+    CK(clwi_mfma_rows(ctx, B->seg_d, 0, B->nrec, B->rowlist, B->mrows));
+    CK(clw_d2h(ctx, rows, B->mrows, 4, 1));
+  }
+  CK(clw_d2h(ctx, desc, B->seg_d, sizeof(SegHost), 1));
+  return CLW_OK;
+}
+
+// The band's product kernels on the trial segment (descriptor desc), timed
+// with events: k_mfma_runs with `slices` row slices (0: none, the run
+// products stay on the vector units) and k_batch_mul, whose work queues
+// start empty. *ms and the factor applications they made.
+static int trial_products(clw_batch* B, Band& bd, const SegHost& desc, uint32_t slices, double* ms, double* apps) {
+  clw_ctx* ctx = B->ctx;
+  CK(clw_h2d(ctx, B->seg_d, &desc, sizeof(SegHost), 0));
+  if (bd.mfma) CK(mfma_plan(B, bd, slices));
+  CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = clw_hip(hipEventCreate(&e0), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventCreate(&e1), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e0, clw_stream(ctx)), "hipEventRecord");
+  if (rc == CLW_OK && slices) {
+    const uint32_t nm = bd.n_mt();
+    rc = clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
+                        bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
+                        B->subm, B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo,
+                        bd.pm_hi, B->totals + 2, bd.mfma_lazy ? 1u : 0u, slices);
+  }
   if (rc == CLW_OK)
     rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,
                         (uint32_t)(bd.L - 2), 0, B->subp, B->subn, B->ds, bd.resident, bd.part_lo, bd.part_hi,
                         B->totals + 2);
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e1, clw_stream(ctx)), "hipEventRecord");
-  uint64_t tot[5] = {0, 0, 0, 0, 0};
+  uint64_t tot[8] = {0};
   if (rc == CLW_OK) rc = clw_d2h(ctx, tot, B->totals, sizeof(tot), 1);
-  float ms = 0;
-  if (rc == CLW_OK) rc = clw_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  float t = 0;
+  if (rc == CLW_OK) rc = clw_hip(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   CK(rc);
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
-  *rate = ms > 0 ? (double)tot[3] / (ms * 1e-3) : 0.0;
+  *ms = t;
+  *apps = (double)tot[3];
+  return CLW_OK;
+}
+
+// Partials of k_mfma_runs for `slices` slices (16 B per slice and test prime).
+static int ensure_pm(clw_batch* B, Band& bd, uint32_t slices) {
+  if (bd.pm_slices >= slices) return CLW_OK;
+  const size_t n = bd.p.size();
+  uint64_t *lo = nullptr, *hi = nullptr;
+  CK(clw_alloc(B->ctx, (size_t)slices * n * 8, (void**)&lo));
+  const int rc = clw_alloc(B->ctx, (size_t)slices * n * 8, (void**)&hi);
+  if (rc != CLW_OK) {
+    clw_free(B->ctx, lo);
+    return rc;
+  }
+  clw_free(B->ctx, bd.pm_lo);
+  clw_free(B->ctx, bd.pm_hi);
+  B->bytes += 16ull * n * (slices - bd.pm_slices);
+  bd.pm_lo = lo;
+  bd.pm_hi = hi;
+  bd.pm_slices = slices;
+  return CLW_OK;
+}
+
+// Step 1 of the tuner for a band with matrix-core run products.
+static int tune_mfma(clw_batch* B, Band& bd, uint64_t q, uint64_t active, TuneChoice* ch, clw_band_tune* t) {
+  std::vector<uint32_t> cand;
+  if (B->opt.mfma_slices) {
+    cand.push_back(B->opt.mfma_slices);
+  } else {
+    cand.push_back(8);
+    // 16 slices: twice the partials (kept below 1 GiB)
+    if (32ull * bd.p.size() * 8 <= (1ull << 30)) cand.push_back(16);
+  }
+  for (uint32_t sl : cand) CK(ensure_pm(B, bd, sl));
+  const uint64_t full = window_blocks(B, q, active);
+  const uint64_t part = std::max<uint64_t>(kMinSpanBlocks, full / 4);
+  double tm[2][4] = {{0}}, tv[2] = {0, 0};
+  uint32_t rows[2] = {0, 0};
+  bool warmed = false;
+  for (int li = 0; li < 2; ++li) {
+    SegHost desc{};
+    CK(trial_segment(B, q, li == 0 ? full : part, &rows[li], &desc));
+    double ms = 0, apps = 0;
+    if (!warmed) {  // code objects, tables, clocks
+      for (uint32_t sl : cand) CK(trial_products(B, bd, desc, sl, &ms, &apps));
+      warmed = true;
+    }
+    CK(trial_products(B, bd, desc, 0, &tv[li], &apps));
+    if (t->mfma_trials < 8) {
+      t->mfma_trial_slices[t->mfma_trials] = 0;
+      t->mfma_trial_rows[t->mfma_trials] = rows[li];
+      t->mfma_trial_ms[t->mfma_trials] = tv[li];
+    }
+    t->mfma_trials++;
+    for (size_t ci = 0; ci < cand.size(); ++ci) {
+      CK(trial_products(B, bd, desc, cand[ci], &tm[li][ci], &apps));
+      if (t->mfma_trials < 8) {
+        t->mfma_trial_slices[t->mfma_trials] = cand[ci];
+        t->mfma_trial_rows[t->mfma_trials] = rows[li];
+        t->mfma_trial_ms[t->mfma_trials] = tm[li][ci];
+      }
+      t->mfma_trials++;
+    }
+  }
+  size_t best = 0;
+  for (size_t ci = 1; ci < cand.size(); ++ci)
+    if (tm[0][ci] < tm[0][best]) best = ci;
+  ch->slices = cand[best];
+  // crossover in rows per slice
+  const double r0 = (double)rows[0] / ch->slices, r1 = (double)rows[1] / ch->slices;
+  double min_rows = 4294967295.0;
+  if (r0 > r1 && r1 > 0) {
+    const double b = (tm[0][best] - tm[1][best]) / (r0 - r1);
+    const double a = std::max(0.0, tm[0][best] - b * r0);
+    const double c = tv[0] / r0;
+    if (c > b) min_rows = a / (c - b);
+  } else if (tm[0][best] < tv[0]) {
+    min_rows = r0;  // one length only: the matrix cores from this segment's rows on
+  }
+  ch->min_rows = (uint32_t)std::min(4294967295.0, std::max(64.0, min_rows));
   return CLW_OK;
 }
 
@@ -1079,6 +1227,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
                                   uint32_t report_cap) {
   if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_autotune: null batch");
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune after iterate");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune with a window in flight");
   clw_ctx* ctx = B->ctx;
   if (q < 7) q = 7;
   if (q > B->maxT) q = B->maxT;
@@ -1098,34 +1247,46 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
     key.cus = ctx->prop.multiProcessorCount;
     key.L = bd.L;
     key.size_class = active ? 63 - __builtin_clzll(active) : 0;
-    uint32_t kp = 0, ipb = 0;
+    TuneChoice ch;
     if (active == 0) {
       // nothing of this band reaches q: keep its plan
-    } else if (cache_lookup(cache_path, key, &kp, &ipb)) {
-      bd.per_lane = kp;
-      bd.items_per_block = ipb;
+      ch = TuneChoice{bd.per_lane, bd.items_per_block, bd.nslices, bd.mfma_min_rows};
+    } else if (cache_lookup(cache_path, key, &ch)) {
       t.from_cache = 1;
+      if (bd.mfma) CK(ensure_pm(B, bd, ch.slices));
     } else {
-      // trial segment of ~2e11 factor applications (a tenth of a step)
-      const double per_block = (double)active * CLW_BLOCK_SPAN / std::log((double)q);
-      const uint64_t blocks = std::min<uint64_t>(B->max_blocks - 2, std::max<uint64_t>(4, (uint64_t)(2e11 / per_block) + 1));
-      // candidates: test primes per lane with a k_batch_mul instantiation
-      static const uint32_t kps_by_band[5][6] = {{0}, {0}, {1, 2, 3, 4}, {2, 3, 4, 5, 6}, {1, 2, 3, 4, 5}};
-      std::vector<uint32_t> kps;
-      for (uint32_t a : kps_by_band[bd.L])
-        if (a) kps.push_back(a);
-      const uint32_t ipbs[3] = {8, 32, 128};
-      double best = -1, warm = 0;
-      bd.per_lane = clw_batch_default_per_lane((uint32_t)bd.L);
+      uint64_t act_all = 0;  // test primes of all bands active at q: they set the window length
+      CK(plan_window(B, q, &act_all));  // the groups and tiles active at q
+      bd.per_lane = B->opt.per_lane ? B->opt.per_lane : clw_batch_default_per_lane((uint32_t)bd.L);
       bd.items_per_block = kItemsPerBlock;
-      CK(tune_trial(B, bd, q, blocks, &warm));  // code-object load, clocks up
-      for (uint32_t a : kps) {
-        if (B->opt.per_lane && a != B->opt.per_lane) continue;
+      CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
+      if (bd.mfma) {
+        CK(tune_mfma(B, bd, q, act_all, &ch, &t));
+      } else {
+        ch.slices = bd.nslices;
+        ch.min_rows = bd.mfma_min_rows;
+      }
+      // step 2 on one window-length segment, the run products where step 1 put them
+      uint32_t rows = 0;
+      SegHost desc{};
+      CK(trial_segment(B, q, window_blocks(B, q, act_all), &rows, &desc));
+      const uint32_t sl = bd.mfma && rows / ch.slices >= ch.min_rows ? ch.slices : 0u;
+      static const uint32_t kps_by_band[5][6] = {{0}, {0}, {1, 2, 3, 4}, {2, 3, 4, 5, 6}, {1, 2, 3, 4, 5}};
+      const uint32_t ipbs[3] = {8, 32, 128};
+      double best = -1, ms = 0, apps = 0;
+      uint64_t act = 0;
+      CK(plan_window(B, q, &act));
+      CK(trial_products(B, bd, desc, sl, &ms, &apps));  // warm-up
+      for (uint32_t a : kps_by_band[bd.L]) {
+        if (!a || (B->opt.per_lane && a != B->opt.per_lane)) continue;
         for (uint32_t b : ipbs) {
           bd.per_lane = a;
           bd.items_per_block = b;
-          double rate = 0;
-          CK(tune_trial(B, bd, q, blocks, &rate));
+          CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
+          if (B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
+          CK(plan_window(B, q, &act));  // items and K for this tile size
+          CK(trial_products(B, bd, desc, sl, &ms, &apps));
+          const double rate = ms > 0 ? apps / (ms * 1e-3) : 0.0;
           if (t.trials < 16) {
             t.trial_per_lane[t.trials] = a;
             t.trial_items_per_block[t.trials] = b;
@@ -1134,24 +1295,30 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
           t.trials++;
           if (rate > best) {
             best = rate;
-            kp = a;
-            ipb = b;
+            ch.kp = a;
+            ch.ipb = b;
           }
         }
       }
-      bd.per_lane = kp;
-      bd.items_per_block = ipb;
       t.best_apps_per_s = best;
-      cache_store(cache_path, key, kp, ipb);
+      cache_store(cache_path, key, ch);
+    }
+    bd.per_lane = ch.kp;
+    bd.items_per_block = ch.ipb;
+    if (bd.mfma) {
+      bd.nslices = ch.slices;
+      bd.mfma_min_rows = ch.min_rows;
     }
     CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
     if (B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
     t.per_lane = bd.per_lane;
     t.items_per_block = bd.items_per_block;
+    t.mfma_slices = bd.mfma ? bd.nslices : 0u;
+    t.mfma_min_rows = bd.mfma ? bd.mfma_min_rows : 0u;
     if (report && nrep < report_cap) report[nrep] = t;
     nrep++;
   }
-  // the product kernels may have changed: capture the segment step again
+  // the product kernels and plans may have changed: capture the segment step again
   if (B->graph_exec) CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
   if (B->graph) CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
   B->graph_exec = nullptr;

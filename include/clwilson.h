@@ -396,24 +396,41 @@ int clw_batch_finalize(clw_batch* b);
  * totalcount is not tracked per range and reads 0) and per-prime outputs. */
 int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, uint64_t* out_p,
                       uint64_t* out_quot, uint64_t* out_rem, uint64_t out_cap);
-/* Autotuner of the product kernel per limb band: times, on one trial
- * segment of sieving primes at magnitude q (sieve + product kernel into
- * scratch; no residue changes), every candidate of test primes per lane
- * (1-4 for 2 limbs, 2-6 for 3, 1-5 for 4 limbs: the number of Montgomery products
- * a lane runs in lockstep, which sets the instruction schedule and the
- * work-group tile of 256 x per_lane test primes) x work items per resident
- * block (8, 32, 128:
- * the chunk count, i.e. the fan-in of the chunk-product reduction), and keeps
- * the fastest. With cache_path, a tuned (device, band, size class) is read
- * from / appended to that text file instead of re-timed. report receives one
- * entry per band present (at most report_cap). No counterpart in the
- * reference, whose launch shapes are fixed (cl_wilson.cpp:1795-1843). */
+/* Autotuner per limb band, at magnitude q (sieving primes near q), on trial
+ * segments (sieve + product kernels into scratch; no residue changes):
+ *  1. a band whose run products can go to the matrix cores (3 limbs; 4 limbs
+ *     below p = 2^47): the row slices of k_mfma_runs (8 or 16; or
+ *     clw_options.mfma_slices) and the switch between the matrix cores and
+ *     the vector units. Both paths are timed on a segment of the length a
+ *     window at q runs and on a quarter of it; with the matrix-core time
+ *     a + b r and the vector time c r in the sub-chunks r per slice, windows
+ *     use the matrix cores from mfma_min_rows = a / (c - b) sub-chunks per
+ *     slice on (never if c <= b). mfma_trial_* hold the trials (slices 0:
+ *     vector units; rows: the segment's sub-chunks).
+ *  2. test primes per lane of the vector product kernel (1-4 for 2 limbs,
+ *     2-6 for 3, 1-5 for 4 limbs: the Montgomery products a lane runs in
+ *     lockstep, which set the instruction schedule and the work-group tile
+ *     of 256 x per_lane test primes) x work items per resident block (8, 32,
+ *     128: the chunk count, i.e. the fan-in of the chunk-product reduction),
+ *     timed with the run products where step 1 put them; the fastest is
+ *     kept.
+ * With cache_path, a tuned (device, band, size class) is read from /
+ * appended to that text file instead of re-timed. report receives one entry
+ * per band present (at most report_cap). The reference sizes its launches
+ * from a timed probe instead (cl_wilson.cpp:1184-1290) and has fixed launch
+ * shapes otherwise (cl_wilson.cpp:1795-1843). */
 typedef struct {
   uint32_t limbs, per_lane, items_per_block, trials, from_cache;
   uint64_t active_test_primes;
   double best_apps_per_s;
   uint32_t trial_per_lane[16], trial_items_per_block[16];
   double trial_apps_per_s[16];
+  uint32_t mfma_slices;   /* chosen row slices of k_mfma_runs (0: band without matrix-core run products) */
+  uint32_t mfma_min_rows; /* sub-chunks per slice and segment from which windows use the matrix cores */
+  uint32_t mfma_trials;
+  uint32_t mfma_trial_slices[8];
+  uint64_t mfma_trial_rows[8];
+  double mfma_trial_ms[8];
 } clw_band_tune;
 int clw_batch_autotune(clw_batch* b, uint64_t q, const char* cache_path, clw_band_tune* report,
                        uint32_t report_cap);

@@ -213,6 +213,7 @@ def test_autotune_keeps_results(dev, tmp_path):
         say("autotune", rep)
         assert rep and all(t["trials"] >= 3 and not t["from_cache"] for t in rep)
         assert all(t["per_lane"] in (1, 2, 3, 4, 5, 6) for t in rep)
+        assert all(rate > 0 for t in rep for _, _, rate in t["candidates"])  # every trial did its work
         assert [b.residues(r) for r in range(len(ranges))] == before
         b.run(1000, b.maxtarget + 1)
         b.iterate()
@@ -224,6 +225,40 @@ def test_autotune_keeps_results(dev, tmp_path):
         assert all(t["from_cache"] for t in rep2)
         assert [(t["per_lane"], t["items_per_block"]) for t in rep2] == \
             [(t["per_lane"], t["items_per_block"]) for t in rep]
+
+
+def test_autotune_matrix_cores_keeps_results(dev, tmp_path):
+    """The matrix-core part of the autotuner (row slices of k_mfma_runs and
+    the switch between the matrix cores and the vector units, both timed on
+    trial segments at p ~ 2e13): its trials leave the residues untouched, a
+    window after tuning equals the same window untuned bit for bit, and a
+    second tuning of the same shape comes from the cache."""
+    rng = [(2 * 10**13, 2 * 10**13 + 1_000_000)]
+    w = (10**12, 10**12 + 2 * 10**9)
+    with cw.Batch(dev, rng) as b:
+        b.run(2, 1000)
+        b.run(*w)
+        ref = b.residues(0)
+    cache = str(tmp_path / "tune.txt")
+    with cw.Batch(dev, rng) as b:
+        b.run(2, 1000)
+        before = b.residues(0)
+        rep = b.autotune(w[0], cache)
+        say("autotune", rep)
+        t = [x for x in rep if x["limbs"] == 3][0]
+        assert t["mfma_slices"] in (8, 16) and t["mfma_min_rows"] >= 64
+        cand = t["mfma_candidates"]
+        assert len(cand) >= 4 and {s for s, _, _ in cand} >= {0, 8} and all(ms > 0 for _, _, ms in cand)
+        assert t["candidates"] and all(rate > 0 for _, _, rate in t["candidates"])  # every trial did its work
+        assert b.residues(0) == before
+        b.run(*w)
+        assert b.residues(0) == ref
+    with cw.Batch(dev, rng) as b:
+        rep2 = b.autotune(w[0], cache)
+        t2 = [x for x in rep2 if x["limbs"] == 3][0]
+        assert t2["from_cache"]
+        assert (t2["per_lane"], t2["items_per_block"], t2["mfma_slices"], t2["mfma_min_rows"]) == \
+            (t["per_lane"], t["items_per_block"], t["mfma_slices"], t["mfma_min_rows"])
 
 
 @pytest.mark.parametrize("per_lane", [1, 2, 3, 4, 5, 6])
