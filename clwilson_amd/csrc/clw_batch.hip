@@ -187,24 +187,36 @@ struct clw_batch {
   } pending;            // window launched by clw_batch_launch, not yet waited for
 
   ~clw_batch() {
-    // teardown: errors here have nobody to report to
+    // teardown: errors here have nobody to report to. The batch's kernels
+    // ran on its stream only (clw_batch_bind drains the old one): once it has
+    // drained, the blocks go back to the cache of idle device memory instead
+    // of hipFree, which would wait for every stream of the device.
+    const bool drained = !ctx || hipStreamSynchronize(clw_stream(ctx)) == hipSuccess;
+    auto give = [&](void* x) {
+      if (drained)
+        (void)clwi_release(ctx, x);
+      else
+        (void)clw_free(ctx, x);
+    };
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (done_ev) (void)hipEventDestroy(done_ev);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (Band& b : bands) {
       if (!b.L) continue;
-      clw_tp_table_free(ctx, &b.tab);
+      void* tab[] = {b.tab.p, b.tab.tp_target, b.tab.n_lo, b.tab.n_hi, b.tab.one_lo, b.tab.one_hi, b.tab.r2_lo,
+                     b.tab.r2_hi, b.tab.acc_lo, b.tab.acc_hi, b.tab.type, b.tab.ninv};
+      for (void* x : tab) give(x);
       void* ptrs[] = {b.groups_d, b.tp_group_d, b.item_gt_d, b.from_d, b.plan_d, b.mt_d, b.pm_lo, b.pm_hi};
-      for (void* x : ptrs) clw_free(ctx, x);
+      for (void* x : ptrs) give(x);
       if (b.ctab) (void)hipFree(b.ctab);
       if (b.part_own) {
-        clw_free(ctx, b.part_lo);
-        clw_free(ctx, b.part_hi);
+        give(b.part_lo);
+        give(b.part_hi);
       }
     }
     void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, mrows, era};
-    for (void* x : ptrs) clw_free(ctx, x);
+    for (void* x : ptrs) give(x);
   }
 };
 

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <unordered_map>
 
 #include "clw_internal.h"
 
@@ -113,23 +114,92 @@ extern "C" int clw_get_device_info(clw_ctx* ctx, clw_device_info* info) {
   return CLW_OK;
 }
 
+// Idle device blocks kept for reuse, per device. hipFree waits for the whole
+// device, so a request loop that retires a batch while other streams run
+// would stall them all, and hipMalloc of large blocks is slow: a batch that
+// is destroyed hands its blocks here instead (clwi_release: its stream has
+// drained, no kernel uses them), and clw_alloc takes a cached block of the
+// same size class before it asks HIP. Sizes go up in steps of 1/8 of a power
+// of two (at most 12.5 % more than asked); at most kCacheCap bytes are kept,
+// and the cache is emptied when HIP runs out of memory.
+namespace {
+constexpr int kMaxDevices = 64;
+constexpr size_t kCacheCap = 16ull << 30;
+std::mutex g_cache_mu;
+std::unordered_multimap<size_t, void*> g_cache[kMaxDevices];  // size class -> idle block
+std::unordered_map<void*, size_t> g_block_size[kMaxDevices];  // every block clw_alloc handed out
+size_t g_cached_bytes[kMaxDevices] = {};
+
+size_t size_class(size_t bytes) {
+  if (bytes <= 256) return 256;
+  const int top = 63 - __builtin_clzll(bytes - 1);  // 2^top < bytes <= 2^(top+1)
+  const size_t step = (size_t)1 << (top - 3);
+  return (bytes + step - 1) / step * step;
+}
+
+void cache_trim(int dev) {  // caller holds g_cache_mu
+  for (auto& kv : g_cache[dev]) {
+    (void)hipFree(kv.second);
+    g_block_size[dev].erase(kv.second);
+  }
+  g_cache[dev].clear();
+  g_cached_bytes[dev] = 0;
+}
+}  // namespace
+
 extern "C" int clw_alloc(clw_ctx* ctx, size_t bytes, void** dptr) {
   if (!ctx || !dptr) return clw_fail(CLW_ERR_ARG, "clw_alloc: null argument");
   *dptr = nullptr;
-  if (bytes == 0) bytes = 16;
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return clw_fail(CLW_ERR_ARG, "clw_alloc: device index");
+  const size_t cls = size_class(bytes == 0 ? 16 : bytes);
   CK_HIP_OR_RETURN(hipSetDevice(ctx->device), "hipSetDevice");
-  const hipError_t e = hipMalloc(dptr, bytes);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_cache[ctx->device].find(cls);
+  if (it != g_cache[ctx->device].end()) {
+    *dptr = it->second;
+    g_cache[ctx->device].erase(it);
+    g_cached_bytes[ctx->device] -= cls;
+    return CLW_OK;
+  }
+  hipError_t e = hipMalloc(dptr, cls);
+  if (e != hipSuccess && g_cached_bytes[ctx->device]) {  // the idle blocks may be in the way
+    (void)hipGetLastError();
+    cache_trim(ctx->device);
+    e = hipMalloc(dptr, cls);
+  }
   if (e != hipSuccess) {
     *dptr = nullptr;
     clw_fail_hip(e, "hipMalloc");
     return CLW_ERR_NOMEM;
   }
+  g_block_size[ctx->device][*dptr] = cls;
   return CLW_OK;
+}
+
+// A block from clw_alloc that no kernel uses any more (the caller has drained
+// every stream that used it) goes to the cache; others are freed.
+extern "C" int clwi_release(clw_ctx* ctx, void* dptr) {
+  if (!dptr) return CLW_OK;
+  if (!ctx || ctx->device < 0 || ctx->device >= kMaxDevices) return clw_free(ctx, dptr);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_block_size[ctx->device].find(dptr);
+    if (it != g_block_size[ctx->device].end() && g_cached_bytes[ctx->device] + it->second <= kCacheCap) {
+      g_cache[ctx->device].emplace(it->second, dptr);
+      g_cached_bytes[ctx->device] += it->second;
+      return CLW_OK;
+    }
+  }
+  return clw_free(ctx, dptr);
 }
 
 extern "C" int clw_free(clw_ctx* ctx, void* dptr) {
   if (!dptr) return CLW_OK;
   if (ctx) CK_HIP_OR_RETURN(hipSetDevice(ctx->device), "hipSetDevice");
+  if (ctx && ctx->device >= 0 && ctx->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_block_size[ctx->device].erase(dptr);
+  }
   const hipError_t e = hipFree(dptr);
   if (e != hipSuccess) return clw_fail_hip(e, "hipFree");
   return CLW_OK;

@@ -110,7 +110,8 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.2,
                  apps_per_second: float = 1e13, segment_blocks: int = 640, max_cohort: int = 64,
-                 admit_interval: float = 0.5, host_primes: bool = True):
+                 admit_interval: float = 0.5, host_primes: bool = True, class_bits: int = 4,
+                 cohort_primes: int = 0):
         self.devices = [Device(device) for _ in range(streams)]
         self.admit_dev = Device(device)
         # chunk products of every cohort on a stream live in that stream's
@@ -125,6 +126,8 @@ class Server:
         self.segment_span = segment_blocks * BLOCK_SPAN
         self.host_primes = host_primes
         self.max_cohort = max_cohort
+        self.class_bits = class_bits          # magnitude classes 2^class_bits wide
+        self.cohort_primes = cohort_primes or self.COHORT_PRIMES  # estimated test primes per cohort at most
         self.admit_interval = admit_interval
         self.lock = threading.Lock()
         self.queue: list[Request] = []
@@ -189,9 +192,10 @@ class Server:
         return int(e)
 
     @staticmethod
-    def magnitude_class(req: Request) -> int:
-        """Classes of the largest factorial target (~pmax/2), a factor 16 wide."""
-        return int(math.log2(max(req.pmax // 2, 2)) / 4)
+    def magnitude_class(req: Request, bits: int = 4) -> int:
+        """Classes of the largest factorial target (~pmax/2), a factor 2^bits
+        wide."""
+        return int(math.log2(max(req.pmax // 2, 2)) / bits)
 
     def admit(self) -> int:
         """One admission round; returns the number of requests admitted."""
@@ -205,10 +209,10 @@ class Server:
         with self.lock:
             while self.queue:
                 r = self.queue[0]
-                k = self.magnitude_class(r)
+                k = self.magnitude_class(r, self.class_bits)
                 grp = classes.get(k)
                 if grp is not None and (len(grp) >= self.max_cohort or
-                                        primes[k] + self.estimate_primes(r) > self.COHORT_PRIMES):
+                                        primes[k] + self.estimate_primes(r) > self.cohort_primes):
                     break
                 e = self.estimate_bytes(r) + (0 if grp else fixed)
                 if self.used + self.reserved + e > self.budget:
