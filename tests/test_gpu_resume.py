@@ -86,6 +86,18 @@ def test_cli_range_results_file_and_done_state(tmp_path):
     assert r.returncode == 0 and "Workunit complete." in r.stdout
 
 
+def test_cli_self_test(tmp_path):
+    """The -s path of the command line: the reference's seven self-test
+    ranges (cl_wilson.cpp:1998-2139) through the binary, each checked
+    against its result count, checksum, pi(maxT), last prime and last w_p."""
+    exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
+    print("clwilson -s", flush=True)
+    r = subprocess.run([exe, "-s"], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "All test cases completed successfully!" in r.stdout
+    assert sum(f"test case {k} passed." in r.stdout for k in range(1, 8)) == 7
+
+
 def test_cli_suffix_arguments(tmp_path):
     exe = os.path.join(ROOT, "clwilson_amd", "clwilson")
     r = subprocess.run([exe, "-p", "86M", "-P", "86010000"], cwd=tmp_path, capture_output=True, text=True,
