@@ -1031,8 +1031,9 @@ extern "C" int clw_batch_kernel_stats(const clw_batch* B, clw_kernel_stats* out)
 //  2. the test primes per lane of k_batch_mul (1, 2 or 4 Montgomery products
 //     in lockstep, i.e. the tile of 256 x KP test primes a work-group
 //     carries) x work items per resident block (the chunk count K, i.e. the
-//     fan-in of the reduction that folds chunk products), timed with the run
-//     products where step 1 put them.
+//     fan-in of the reduction that folds chunk products), timed on the work
+//     step 1 leaves it (where the matrix cores take the run products, only
+//     the records across run boundaries).
 // Trials write chunk products and partials to scratch and run no reduction,
 // so no residue changes. Results can be kept in a text cache keyed by device,
 // band and the log2 of the band's active test primes.
@@ -1119,9 +1120,12 @@ static int trial_segment(clw_batch* B, uint64_t q, uint64_t span_blocks, uint32_
 
 // The band's product kernels on the trial segment (descriptor desc), timed
 // with events: k_mfma_runs with `slices` row slices (0: none, the run
-// products stay on the vector units) and k_batch_mul, whose work queues
-// start empty. *ms and the factor applications they made.
-static int trial_products(clw_batch* B, Band& bd, const SegHost& desc, uint32_t slices, double* ms, double* apps) {
+// products stay on the vector units; with_mfma false: the plan only, so
+// k_batch_mul skips what the matrix cores would take, and k_mfma_runs, whose
+// time does not depend on k_batch_mul's shape, is not run) and k_batch_mul,
+// whose work queues start empty. *ms and the factor applications they made.
+static int trial_products(clw_batch* B, Band& bd, const SegHost& desc, uint32_t slices, bool with_mfma, double* ms,
+                          double* apps) {
   clw_ctx* ctx = B->ctx;
   CK(clw_h2d(ctx, B->seg_d, &desc, sizeof(SegHost), 0));
   if (bd.mfma) CK(mfma_plan(B, bd, slices));
@@ -1130,7 +1134,7 @@ static int trial_products(clw_batch* B, Band& bd, const SegHost& desc, uint32_t 
   int rc = clw_hip(hipEventCreate(&e0), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventCreate(&e1), "hipEventCreate");
   if (rc == CLW_OK) rc = clw_hip(hipEventRecord(e0, clw_stream(ctx)), "hipEventRecord");
-  if (rc == CLW_OK && slices) {
+  if (rc == CLW_OK && slices && with_mfma) {
     const uint32_t nm = bd.n_mt();
     rc = clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                         bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
@@ -1196,10 +1200,10 @@ static int tune_mfma(clw_batch* B, Band& bd, uint64_t q, uint64_t active, TuneCh
     CK(trial_segment(B, q, li == 0 ? full : part, &rows[li], &desc));
     double ms = 0, apps = 0;
     if (!warmed) {  // code objects, tables, clocks
-      for (uint32_t sl : cand) CK(trial_products(B, bd, desc, sl, &ms, &apps));
+      for (uint32_t sl : cand) CK(trial_products(B, bd, desc, sl, true, &ms, &apps));
       warmed = true;
     }
-    CK(trial_products(B, bd, desc, 0, &tv[li], &apps));
+    CK(trial_products(B, bd, desc, 0, true, &tv[li], &apps));
     if (t->mfma_trials < 8) {
       t->mfma_trial_slices[t->mfma_trials] = 0;
       t->mfma_trial_rows[t->mfma_trials] = rows[li];
@@ -1207,7 +1211,7 @@ static int tune_mfma(clw_batch* B, Band& bd, uint64_t q, uint64_t active, TuneCh
     }
     t->mfma_trials++;
     for (size_t ci = 0; ci < cand.size(); ++ci) {
-      CK(trial_products(B, bd, desc, cand[ci], &tm[li][ci], &apps));
+      CK(trial_products(B, bd, desc, cand[ci], true, &tm[li][ci], &apps));
       if (t->mfma_trials < 8) {
         t->mfma_trial_slices[t->mfma_trials] = cand[ci];
         t->mfma_trial_rows[t->mfma_trials] = rows[li];
@@ -1285,10 +1289,10 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
       const uint32_t sl = bd.mfma && rows / ch.slices >= ch.min_rows ? ch.slices : 0u;
       static const uint32_t kps_by_band[5][6] = {{0}, {0}, {1, 2, 3, 4}, {2, 3, 4, 5, 6}, {1, 2, 3, 4, 5}};
       const uint32_t ipbs[3] = {8, 32, 128};
-      double best = -1, ms = 0, apps = 0;
+      double best = -1, best_ms = 1e300, ms = 0, apps = 0;  // every candidate on the same segment: by time
       uint64_t act = 0;
       CK(plan_window(B, q, &act));
-      CK(trial_products(B, bd, desc, sl, &ms, &apps));  // warm-up
+      CK(trial_products(B, bd, desc, sl, false, &ms, &apps));  // warm-up
       for (uint32_t a : kps_by_band[bd.L]) {
         if (!a || (B->opt.per_lane && a != B->opt.per_lane)) continue;
         for (uint32_t b : ipbs) {
@@ -1297,7 +1301,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
           CK(clwi_batch_resident(ctx, (uint32_t)bd.L, bd.per_lane, &bd.resident));
           if (B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
           CK(plan_window(B, q, &act));  // items and K for this tile size
-          CK(trial_products(B, bd, desc, sl, &ms, &apps));
+          CK(trial_products(B, bd, desc, sl, false, &ms, &apps));
           const double rate = ms > 0 ? apps / (ms * 1e-3) : 0.0;
           if (t.trials < 16) {
             t.trial_per_lane[t.trials] = a;
@@ -1305,7 +1309,8 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
             t.trial_apps_per_s[t.trials] = rate;
           }
           t.trials++;
-          if (rate > best) {
+          if (ms < best_ms) {
+            best_ms = ms;
             best = rate;
             ch.kp = a;
             ch.ipb = b;

@@ -412,8 +412,11 @@ int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, ui
  *     lockstep, which set the instruction schedule and the work-group tile
  *     of 256 x per_lane test primes) x work items per resident block (8, 32,
  *     128: the chunk count, i.e. the fan-in of the chunk-product reduction),
- *     timed with the run products where step 1 put them; the fastest is
- *     kept.
+ *     timed on the work step 1 leaves to that kernel (where the matrix cores
+ *     take the run products, the records across run boundaries only:
+ *     best_apps_per_s and trial_apps_per_s are then that kernel's own
+ *     factor applications per second, 0 for a segment with no record
+ *     across a run boundary); the fastest on the trial segment is kept.
  * With cache_path, a tuned (device, band, size class) is read from /
  * appended to that text file instead of re-timed. report receives one entry
  * per band present (at most report_cap). The reference sizes its launches

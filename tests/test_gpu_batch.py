@@ -249,7 +249,8 @@ def test_autotune_matrix_cores_keeps_results(dev, tmp_path):
         assert t["mfma_slices"] in (8, 16) and t["mfma_min_rows"] >= 64
         cand = t["mfma_candidates"]
         assert len(cand) >= 4 and {s for s, _, _ in cand} >= {0, 8} and all(ms > 0 for _, _, ms in cand)
-        assert t["candidates"] and all(rate > 0 for _, _, rate in t["candidates"])  # every trial did its work
+        # per-lane trials time the vector kernel alone on what the matrix cores leave it
+        assert t["candidates"] and all(rate >= 0 for _, _, rate in t["candidates"])
         assert b.residues(0) == before
         b.run(*w)
         assert b.residues(0) == ref
