@@ -656,7 +656,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
     CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                       bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq, B->subm,
                       B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo, bd.pm_hi,
-                      B->totals + 2, bd.mfma_lazy ? 1u : 0u, bd.nslices));
+                      B->totals + 2, bd.mfma_lazy ? 1u : 0u, bd.nslices, B->seg_d));
   }
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
@@ -1139,7 +1139,7 @@ static int trial_products(clw_batch* B, Band& bd, const SegHost& desc, uint32_t 
     rc = clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                         bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
                         B->subm, B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo,
-                        bd.pm_hi, B->totals + 2, bd.mfma_lazy ? 1u : 0u, slices);
+                        bd.pm_hi, B->totals + 2, bd.mfma_lazy ? 1u : 0u, slices, B->seg_d);
   }
   if (rc == CLW_OK)
     rc = clwi_batch_mul(ctx, &bd.tab, bd.per_lane, bd.groups_d, bd.item_gt_d, bd.plan_d, B->rec, B->nrec, B->seg_d,

@@ -78,7 +78,7 @@ int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, cons
                    const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active, const uint32_t* n_active,
                    uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows, const void* subq,
                    const uint32_t* subm, const int8_t* subb, uint64_t subb_bytes, const void* ctab, uint64_t* pm_lo,
-                   uint64_t* pm_hi, uint64_t* apps, uint32_t lazy, uint32_t nslices);
+                   uint64_t* pm_hi, uint64_t* apps, uint32_t lazy, uint32_t nslices, const void* seg);
 int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uint32_t* nrec, uint32_t nblk, uint32_t* subp,
                          uint32_t* subn, uint32_t ds, uint32_t* flags);
 int clwi_mul_records_runs(clw_ctx* ctx, const clw_tp_table* t, const clw_seg_buffers* s, const uint32_t* subp,

@@ -1476,7 +1476,7 @@ extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* g
                               const uint32_t* n_active, uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows,
                               const void* subq, const uint32_t* subm, const int8_t* subb, uint64_t subb_bytes,
                               const void* ctab, uint64_t* pm_lo, uint64_t* pm_hi, uint64_t* apps, uint32_t lazy,
-                              uint32_t nslices) {
+                              uint32_t nslices, const void* seg) {
   if (t->limbs != 3 && t->limbs != 4) return clw_fail(CLW_ERR_ARG, "matrix-core run products: 3 or 4 limbs");
   if (n_mt == 0) return CLW_OK;
   if (nslices % kXcds || nslices == 0 || nslices > kMfmaMaxSlices)
@@ -1487,7 +1487,7 @@ extern "C" int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* g
   auto launch = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(kMfmaThreads), 0, clw_stream(ctx), CTX_ARGS(t), (const BatchGroup*)groups,
                        mt_group, mt_first, mt_n, mt_active, n_active, rowlist, mrows, (const uint4*)subq, subm, subb,
-                       (uint32_t)subb_bytes, (const uint4*)ctab, t->count, nslices, pm_lo, pm_hi,
+                       (uint32_t)subb_bytes, (const uint4*)ctab, t->count, nslices, (const BatchSeg*)seg, pm_lo, pm_hi,
                        (unsigned long long*)apps);
   };
   if (t->limbs == 4)  // N < 2^94 (the batch takes p < 2^47 here): the lazy chain always holds
