@@ -42,10 +42,7 @@ def test_matrix_core_windows_vs_oracle(dev, golden, mode):
             assert b.info[0].limbs == (3 if w["pmax"] < 24_880_000_000_000 else 4)
             assert b.info[0].tpcount >= 32
             assert list(b.info[0].typeTarget) == [w["type_targets"][str(t)] for t in range(3)]
-            # one window as given (launch/wait: run would split it at the
-            # type target inside, into parts too short for the matrix cores)
-            b.launch(w["qlo"], w["qhi"])
-            b.wait()
+            b.run(w["qlo"], w["qhi"])
             ks = b.kernel_stats()
             assert ks["mfma_int8_macs"] > 0, "the window did not reach the matrix cores"
             assert b.residues(0) == [int(v) for v in w["value"]]
@@ -62,8 +59,7 @@ def test_matrix_cores_stop_at_2_47(dev):
         for mode in (0, 1):
             with cw.Batch(dev, [(lo, lo + 1800)], run_products=mode) as b:
                 assert b.info[0].limbs == 4
-                b.launch(*q)  # one window across the type-1 target (run would split it there)
-                b.wait()
+                b.run(*q)
                 out.append(b.residues(0))
                 macs = b.kernel_stats()["mfma_int8_macs"]
                 say("p", lo, "mode", mode, "int8 MACs", macs)
