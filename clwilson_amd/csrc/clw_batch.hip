@@ -157,10 +157,11 @@ struct clw_batch {
   uint32_t* subp = nullptr;    // run products: digits of each (block, sub-chunk)
   uint32_t* subn = nullptr;    // their digit counts
   uint32_t ds = 0;             // digit stride (a multiple of 12)
-  int8_t* subb = nullptr;      // run products as kMfmaKB signed bytes (for a band with mfma)
+  int8_t* subb = nullptr;      // run products as kMfmaKB signed bytes, in row blocks (for a band with mfma)
   void* subq = nullptr;         // per sub-chunk: first and last prime (uint4)
   uint32_t* subm = nullptr;     // per sub-chunk: primes | bytes << 16
   uint32_t* rowlist = nullptr;  // the segment's sub-chunks, dense (k_mfma_rows)
+  uint32_t* rowpos = nullptr;   // and each sub-chunk's place in that list
   uint32_t* era = nullptr;      // wheel bitmap of the sieve of Eratosthenes (segments below 2^50)
   uint32_t* mrows = nullptr;
   bool any_mfma = false;
@@ -215,10 +216,15 @@ struct clw_batch {
         give(b.part_hi);
       }
     }
-    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, mrows, era};
+    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, rowpos, mrows, era};
     for (void* x : ptrs) give(x);
   }
 };
+
+// bytes of the run products' row buffer: every sub-chunk slot of the record
+// buffer as a row, in whole row blocks, and the step k_mfma_runs loads past
+// the last one
+static size_t subb_bytes(const clw_batch* B) { return mfma_rows_bytes((size_t)B->max_blocks * kBatchSubCap); }
 
 extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64_t* pmax, uint32_t nranges,
                                 const clw_options* opt, clw_batch** out) {
@@ -374,11 +380,11 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     B->any_mfma = true;
   }
   if (B->any_mfma) {
-    // + 1 KB: k_mfma_runs reads up to one row past its last row (unused bytes)
-    if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * kMfmaKB + 1024, (void**)&B->subb);
+    if (rc == CLW_OK) rc = balloc(subb_bytes(B), (void**)&B->subb);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 16, &B->subq);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subm);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->rowlist);
+    if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->rowpos);
     if (rc == CLW_OK) rc = balloc(4, (void**)&B->mrows);
   }
   if (rc == CLW_OK) rc = clw_memset(ctx, B->flags, 0, 16);
@@ -645,9 +651,10 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
   CK(mark(B, timed));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals, B->era, B->maxT + 1));
+  // the row list first: the run products' bytes go to their rows
+  if (B->any_mfma) CK(clwi_mfma_rows(ctx, B->seg_d, extra_block, B->nrec, B->rowlist, B->rowpos, B->mrows));
   CK(clwi_batch_subprod(ctx, B->seg_d, extra_block, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
-                        B->any_mfma ? B->subb : nullptr, B->subq, B->subm, B->flags));
-  if (B->any_mfma) CK(clwi_mfma_rows(ctx, B->seg_d, extra_block, B->nrec, B->rowlist, B->mrows));
+                        B->any_mfma ? B->subb : nullptr, B->rowpos, B->subq, B->subm, B->flags));
   CK(mark(B, timed));
   for (int band = 2; band <= 4; ++band) {
     Band& bd = B->bands[band];
@@ -655,7 +662,7 @@ static int batch_step(clw_batch* B, uint32_t extra_block, bool timed) {
     const uint32_t nm = bd.n_mt();
     CK(clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                       bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq, B->subm,
-                      B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo, bd.pm_hi,
+                      B->subb, subb_bytes(B), bd.ctab, bd.pm_lo, bd.pm_hi,
                       B->totals + 2, bd.mfma_lazy ? 1u : 0u, bd.nslices, B->seg_d));
   }
   CK(mark(B, timed));
@@ -1107,13 +1114,11 @@ static int trial_segment(clw_batch* B, uint64_t q, uint64_t span_blocks, uint32_
   CK(clw_memset(ctx, B->totals, 0, kTotalsBytes));
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
   CK(clwi_batch_sieve(ctx, B->seg_d, B->rec, B->nrec, B->max_blocks - 1, B->flags, B->totals, B->era, B->maxT + 1));
-  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
-                        B->any_mfma ? B->subb : nullptr, B->subq, B->subm, B->flags));
   *rows = 0;
-  if (B->any_mfma) {
-    CK(clwi_mfma_rows(ctx, B->seg_d, 0, B->nrec, B->rowlist, B->mrows));
-    CK(clw_d2h(ctx, rows, B->mrows, 4, 1));
-  }
+  if (B->any_mfma) CK(clwi_mfma_rows(ctx, B->seg_d, 0, B->nrec, B->rowlist, B->rowpos, B->mrows));
+  CK(clwi_batch_subprod(ctx, B->seg_d, 0, B->rec, B->nrec, B->max_blocks, B->subp, B->subn, B->ds,
+                        B->any_mfma ? B->subb : nullptr, B->rowpos, B->subq, B->subm, B->flags));
+  if (B->any_mfma) CK(clw_d2h(ctx, rows, B->mrows, 4, 1));
   CK(clw_d2h(ctx, desc, B->seg_d, sizeof(SegHost), 1));
   return CLW_OK;
 }
@@ -1138,7 +1143,7 @@ static int trial_products(clw_batch* B, Band& bd, const SegHost& desc, uint32_t 
     const uint32_t nm = bd.n_mt();
     rc = clwi_mfma_runs(ctx, &bd.tab, bd.groups_d, bd.mt_d, bd.mt_d + nm, bd.mt_d + 2 * (size_t)nm,
                         bd.mt_d + 3 * (size_t)nm, bd.mt_d + 4 * (size_t)nm, nm, B->rowlist, B->mrows, B->subq,
-                        B->subm, B->subb, (uint64_t)B->max_blocks * kBatchSubCap * kMfmaKB, bd.ctab, bd.pm_lo,
+                        B->subm, B->subb, subb_bytes(B), bd.ctab, bd.pm_lo,
                         bd.pm_hi, B->totals + 2, bd.mfma_lazy ? 1u : 0u, slices, B->seg_d);
   }
   if (rc == CLW_OK)

@@ -52,6 +52,19 @@ constexpr uint32_t kMfmaKB = 768;    // bytes per product: 128 primes below 2^46
 constexpr uint32_t kMfmaSlices = 8;      // fewest slices (one per XCD)
 constexpr uint32_t kMfmaMaxSlices = 32;  // most: a slice's rows then stay in one XCD's L2
 constexpr size_t kMfmaTileBytes = (size_t)kMfmaCols * kMfmaKB * kMfmaTile;  // 147,456
+// The bytes of the segment's sub-chunk products sit in the order the matrix
+// cores read them: row r (its place in the segment's row list) belongs to
+// row block r / 16; a row block holds, for each 64-byte step kb, the 1 KB
+// operand of v_mfma_i32_16x16x64_i8 in lane order (lane 16 g + t: bytes
+// [64 kb + 16 g, + 16) of row 16 (r / 16) + t). One wave's load of a step is
+// then one contiguous KB.
+constexpr uint32_t kMfmaRowBlockBytes = 16 * kMfmaKB;  // 12,288
+__host__ __device__ constexpr size_t mfma_row_byte(uint32_t r, uint32_t i) {
+  return (size_t)(r / 16) * kMfmaRowBlockBytes + (size_t)(i / 64) * 1024 + ((i / 16) % 4 * 16 + r % 16) * 16 + i % 16;
+}
+// bytes of the row buffer for `rows` rows: whole row blocks and the 1 KB step
+// that k_mfma_runs loads past the last one (unused)
+constexpr size_t mfma_rows_bytes(size_t rows) { return (rows + 15) / 16 * kMfmaRowBlockBytes + 1024; }
 
 // Batched-engine launchers (clw_kernels.hip) used by clw_batch.hip.
 extern "C" {
@@ -65,12 +78,12 @@ int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_lane, const
                    uint32_t qslot, uint32_t extra_block, const uint32_t* subp, const uint32_t* subn, uint32_t ds,
                    uint32_t grid, uint64_t* part_lo, uint64_t* part_hi, uint64_t* apps);
 int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec, const uint32_t* nrec,
-                       uint32_t max_blocks, uint32_t* subp, uint32_t* subn, uint32_t ds, int8_t* subb, void* subq,
-                       uint32_t* subm, uint32_t* flags);
+                       uint32_t max_blocks, uint32_t* subp, uint32_t* subn, uint32_t ds, int8_t* subb,
+                       const uint32_t* rowpos, void* subq, uint32_t* subm, uint32_t* flags);
 int clwi_mfma_table(clw_ctx* ctx, const clw_tp_table* t, const uint32_t* mt_first, const uint32_t* mt_n,
                     uint32_t n_mt, void* ctab);
 int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* nrec, uint32_t* rowlist,
-                   uint32_t* mrows);
+                   uint32_t* rowpos, uint32_t* mrows);
 // Returns a block from clw_alloc that no kernel uses any more (every stream
 // that used it has drained) to the device's cache of idle blocks.
 int clwi_release(clw_ctx* ctx, void* dptr);

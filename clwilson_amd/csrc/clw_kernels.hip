@@ -1272,12 +1272,14 @@ extern "C" int clwi_batch_mul(clw_ctx* ctx, const clw_tp_table* t, uint32_t per_
 // One thread per (block, sub-chunk) slot of the record buffer.
 extern "C" int clwi_batch_subprod(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* rec,
                                   const uint32_t* nrec, uint32_t max_blocks, uint32_t* subp, uint32_t* subn,
-                                  uint32_t ds, int8_t* subb, void* subq, uint32_t* subm, uint32_t* flags) {
+                                  uint32_t ds, int8_t* subb, const uint32_t* rowpos, void* subq, uint32_t* subm,
+                                  uint32_t* flags) {
   if (ds % 12 || ds > kBatchSubMaxDigits) return clw_fail(CLW_ERR_ARG, "run products: bad digit stride %u", ds);
+  if (subb && !rowpos) return clw_fail(CLW_ERR_ARG, "run products as bytes need their row positions");
   const uint32_t threads = max_blocks * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
-                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, subb, (uint4*)subq, subm,
-                     flags);
+                     (const BatchSeg*)seg, extra_block, 0u, rec, nrec, subp, subn, ds, subb, rowpos, (uint4*)subq,
+                     subm, flags);
   return clw_check_launch("subchunk_prod");
 }
 
@@ -1289,7 +1291,7 @@ extern "C" int clwi_session_subprod(clw_ctx* ctx, const uint32_t* rec, const uin
   const uint32_t threads = nblk * kBatchSubCap;
   hipLaunchKernelGGL(k_subchunk_prod, dim3(ceil_div(threads, kSubThreads)), dim3(kSubThreads), 0, clw_stream(ctx),
                      (const BatchSeg*)nullptr, 0u, nblk, rec, nrec, subp, subn, ds, (int8_t*)nullptr,
-                     (uint4*)nullptr, (uint32_t*)nullptr, flags);
+                     (const uint32_t*)nullptr, (uint4*)nullptr, (uint32_t*)nullptr, flags);
   return clw_check_launch("subchunk_prod");
 }
 
@@ -1462,9 +1464,9 @@ extern "C" int clwi_mfma_table(clw_ctx* ctx, const clw_tp_table* t, const uint32
 }
 
 extern "C" int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t* nrec, uint32_t* rowlist,
-                              uint32_t* mrows) {
+                              uint32_t* rowpos, uint32_t* mrows) {
   hipLaunchKernelGGL(k_mfma_rows, dim3(1), dim3(1024), 0, clw_stream(ctx), (const BatchSeg*)seg, extra_block, nrec,
-                     rowlist, mrows);
+                     rowlist, rowpos, mrows);
   return clw_check_launch("mfma_rows");
 }
 
