@@ -207,6 +207,8 @@ def lib():
         "clw_batch_covered": (u64, [vp]),
         "clw_batch_device_bytes": (u64, [vp]),
         "clw_batch_finish_range": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
+        "clw_batch_finish_launch": (i32, [vp, u32]),
+        "clw_batch_finish_query": (i32, [vp, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -247,6 +249,19 @@ def host_primes(lo: int, hi: int) -> np.ndarray:
 def format_result(p: int, quot: int) -> str | None:
     buf = C.create_string_buffer(128)
     return buf.value.decode() if lib().clw_format_result(p, quot, buf, 128) else None
+
+
+def result_lines(p: np.ndarray, quot: np.ndarray) -> list[str]:
+    """results.txt lines of the Wilson and near-Wilson primes among p (in
+    order): clw_format_result on the entries with |w_p| < 1000, the only ones
+    it formats."""
+    near = np.nonzero(np.minimum(quot, p - quot) < 1000)[0]
+    out = []
+    for i in near.tolist():
+        line = format_result(int(p[i]), int(quot[i]))
+        if line:
+            out.append(line)
+    return out
 
 
 GOOD_RESULTS_SUM = 0x6659912234F44428  # the reference's goodWilsonResults.txt (cl_wilson.cpp:970)
@@ -355,10 +370,7 @@ class Device:
             check(rc, f"wilson_range [{pmin}, {pmax})")
         res = RangeResult(pmin, pmax, p[:n], q[:n], r[:n], st.checksum, st.totalcount, list(st.typeTarget),
                           list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs, st.seconds)
-        for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
-            line = format_result(pi, qi)
-            if line:
-                res.results.append(line)
+        res.results = result_lines(res.p, res.quot)
         return res
 
 
@@ -447,10 +459,7 @@ class Session:
         res = RangeResult(self.pmin, self.pmax, p[:n], q[:n], r[:n], st.checksum, st.totalcount,
                           list(st.typeTarget), list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs,
                           st.seconds)
-        for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
-            line = format_result(pi, qi)
-            if line:
-                res.results.append(line)
+        res.results = result_lines(res.p, res.quot)
         return res
 
 
@@ -560,8 +569,21 @@ class Batch:
 
     def finish_range(self, r: int, allow_check_failure: bool = False) -> RangeResult:
         """Iterate and check range r alone once its sieving primes are all
-        applied (covered() > its largest target); the rest keep running."""
+        applied (covered() > its largest target); the rest keep running.
+        Collects the results of finish_launch(r) if that was called first."""
         return self._results(r, lib().clw_batch_finish_range, allow_check_failure)
+
+    def finish_launch(self, r: int) -> None:
+        """finish_range in the background: enqueues range r's iterate and
+        check (and the copy of its results) on the batch's stream and returns;
+        finish_ready tells when finish_range will collect without waiting."""
+        check(lib().clw_batch_finish_launch(self.h, r), f"batch_finish_launch {r}")
+
+    def finish_ready(self, r: int) -> bool:
+        rc = lib().clw_batch_finish_query(self.h, r)
+        if rc < 0:
+            check(rc, f"batch_finish_query {r}")
+        return rc == 1
 
     def applied(self) -> int:
         return lib().clw_batch_applied(self.h)
@@ -602,10 +624,7 @@ class Batch:
         pmin, pmax = self.ranges[r]
         res = RangeResult(pmin, pmax, p[:n], q[:n], m[:n], st.checksum, st.totalcount, list(st.typeTarget),
                           list(st.tpcnt), st.resultcount, st.lastp, st.lastw, st.limbs, st.seconds)
-        for pi, qi in zip(res.p.tolist(), res.quot.tolist()):
-            line = format_result(pi, qi)
-            if line:
-                res.results.append(line)
+        res.results = result_lines(res.p, res.quot)
         return res
 
     def search(self) -> list[RangeResult]:

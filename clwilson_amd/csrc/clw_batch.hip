@@ -49,6 +49,7 @@ const uint32_t kItemsPerBlock = 32;    // work items per resident block (balance
 const uint32_t kMaxChunks = 256;
 const uint64_t kTableBytesPerPrime = 88;  // clw_tp_table arrays per test prime
 const size_t kTotalsBytes = 64;            // device counters (8 words)
+const size_t kStageBack = 128;             // staging bytes for the counters handed back (flags 16, totals 64)
 const uint32_t LB_DIGIT = 30;              // bits per digit of a run product
 const uint64_t kMfmaTableCap = 8ull << 30;  // byte tables kept in HBM up to this size
 const uint64_t kMfmaMaxP4 = 1ull << 47;   // 4-limb band on the matrix cores below this p
@@ -107,6 +108,12 @@ struct RangeHost {
   uint32_t band_off = 0;      // its test primes are [band_off, band_off + tp.size()) of the band table
   bool finished = false;      // iterated and checked on its own (clw_batch_finish_range)
   std::vector<uint64_t> quot, rem;  // by band-table order, once finished
+  // a finish in flight (clw_batch_finish_launch): quotients then remainders
+  // on the device, their pinned host copy, and the event after the copy
+  bool finishing = false;
+  uint64_t* fin_d = nullptr;
+  uint64_t* fin_h = nullptr;
+  hipEvent_t fin_ev = nullptr;
 };
 
 // The table restricted to entries [off, off + n) (same arrays, offset).
@@ -180,6 +187,20 @@ struct clw_batch {
   double wall_ms = 0;  // host wall time of clw_batch_run calls
   uint64_t next_q = 2;  // sieving primes [2, next_q) applied to every range
   uint64_t bytes = 0;   // device memory held
+  // Pinned staging for a window's uploads (plans, work lists, the segment
+  // descriptor) and for the counters it hands back: a copy from pageable
+  // memory blocks the host until the device runs it, which, with other
+  // streams' kernels holding every CU, can take a whole window of theirs.
+  // Bytes [0, kStageBack) take the flags and counters of the last window;
+  // uploads go after them, and stage_ev follows the last one.
+  uint8_t* stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
+  hipEvent_t stage_ev = nullptr;
+  bool back_staged = false;  // the last window's flags and counters are copied into stage
+  // creation: the tables' uploads (pinned, released once ready_ev has passed)
+  // and the event after them and the setup kernels
+  uint8_t* create_stage = nullptr;
+  hipEvent_t ready_ev = nullptr;
   struct {
     bool active = false;
     uint64_t qlo = 0, qhi = 0;
@@ -187,12 +208,25 @@ struct clw_batch {
     std::chrono::steady_clock::time_point t0;
   } pending;            // window launched by clw_batch_launch, not yet waited for
 
+  // Waits for the batch's own device work, not for its stream (which other
+  // batches may share): every call that enqueues work either waits for it
+  // before it returns or ends it with an event (done_ev after a window,
+  // fin_ev after a finish), so those events cover all of it.
+  // (with_ready false: creation's work, ready_ev, is left to the caller)
+  bool quiesce(bool with_ready = true) const {
+    bool ok = !done_ev || hipEventSynchronize(done_ev) == hipSuccess;
+    ok &= !stage_ev || hipEventSynchronize(stage_ev) == hipSuccess;
+    ok &= !with_ready || !ready_ev || hipEventSynchronize(ready_ev) == hipSuccess;
+    for (const RangeHost& R : ranges)
+      if (R.fin_ev) ok &= hipEventSynchronize(R.fin_ev) == hipSuccess;
+    return ok;
+  }
+
   ~clw_batch() {
-    // teardown: errors here have nobody to report to. The batch's kernels
-    // ran on its stream only (clw_batch_bind drains the old one): once it has
-    // drained, the blocks go back to the cache of idle device memory instead
+    // teardown: errors here have nobody to report to. Once the batch's work
+    // is done, its blocks go back to the cache of idle device memory instead
     // of hipFree, which would wait for every stream of the device.
-    const bool drained = !ctx || hipStreamSynchronize(clw_stream(ctx)) == hipSuccess;
+    const bool drained = !ctx || quiesce();
     auto give = [&](void* x) {
       if (drained)
         (void)clwi_release(ctx, x);
@@ -201,6 +235,19 @@ struct clw_batch {
     };
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (done_ev) (void)hipEventDestroy(done_ev);
+    if (stage_ev) (void)hipEventDestroy(stage_ev);
+    if (ready_ev) (void)hipEventDestroy(ready_ev);
+    if (drained) {  // else a copy may still read them
+      clwi_host_release(stage);
+      clwi_host_release(create_stage);
+    }
+    for (RangeHost& R : ranges) {
+      if (R.finishing) {
+        give(R.fin_d);
+        clwi_host_release(R.fin_h);
+      }
+      if (R.fin_ev) (void)hipEventDestroy(R.fin_ev);
+    }
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (Band& b : bands) {
@@ -225,6 +272,29 @@ struct clw_batch {
 // buffer as a row, in whole row blocks, and the step k_mfma_runs loads past
 // the last one
 static size_t subb_bytes(const clw_batch* B) { return mfma_rows_bytes((size_t)B->max_blocks * kBatchSubCap); }
+
+// An upload through the pinned staging area: the bytes are copied there and
+// the device copy is enqueued without blocking the host (a blocking copy
+// from the source when the area is full or missing). Uploads only append
+// until stage_reset, which first waits for the ones before it (normally long
+// done: they precede the last window's kernels).
+static int stage_h2d(clw_batch* B, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return CLW_OK;
+  if (!B->stage || B->stage_off + bytes > B->stage_cap) return clw_h2d(B->ctx, dst, src, bytes, 1);
+  uint8_t* const at = B->stage + B->stage_off;
+  memcpy(at, src, bytes);
+  CK(clw_hip(hipMemcpyAsync(dst, at, bytes, hipMemcpyHostToDevice, clw_stream(B->ctx)), "staged upload"));
+  if (!B->stage_ev) CK(clw_hip(hipEventCreateWithFlags(&B->stage_ev, hipEventDisableTiming), "hipEventCreate"));
+  CK(clw_hip(hipEventRecord(B->stage_ev, clw_stream(B->ctx)), "hipEventRecord"));
+  B->stage_off += (bytes + 15) & ~(size_t)15;
+  return CLW_OK;
+}
+
+static int stage_reset(clw_batch* B) {
+  if (B->stage_ev) CK(clw_hip(hipEventSynchronize(B->stage_ev), "hipEventSynchronize"));
+  B->stage_off = kStageBack;
+  return CLW_OK;
+}
 
 extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64_t* pmax, uint32_t nranges,
                                 const clw_options* opt, clw_batch** out) {
@@ -389,6 +459,16 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
   }
   if (rc == CLW_OK) rc = clw_memset(ctx, B->flags, 0, 16);
   if (rc == CLW_OK) rc = clw_memset(ctx, B->totals, 0, kTotalsBytes);
+  size_t stage_bytes = kStageBack + 2 * (sizeof(SegHost) + 16) + 4096;  // the staging area (two windows' uploads)
+  // the tables' uploads, enqueued together after the loop from one pinned
+  // block (a copy from pageable memory would block until the device runs it)
+  struct Upload {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  std::vector<Upload> uploads;
+  std::vector<std::vector<uint32_t>> upload_words;  // sources that live only here
   for (Band& bd : B->bands) {
     if (!bd.L || rc != CLW_OK) continue;
     bd.per_lane = B->opt.per_lane ? B->opt.per_lane : clw_batch_default_per_lane((uint32_t)bd.L);
@@ -405,16 +485,17 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     if (rc == CLW_OK && B->opt.target_waves) bd.resident = std::max(1u, B->opt.target_waves / 4);
     if (rc == CLW_OK) rc = clw_tp_table_alloc(ctx, n, (uint32_t)bd.L, &bd.tab);
     B->bytes += (uint64_t)n * kTableBytesPerPrime;
-    if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tab.p, bd.p.data(), (size_t)n * 8, 1);
-    if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tab.type, bd.type.data(), (size_t)n * 4, 1);
-    if (rc == CLW_OK) rc = clw_setup(ctx, &bd.tab, 0);
+    if (rc == CLW_OK) {
+      uploads.push_back({bd.tab.p, bd.p.data(), (size_t)n * 8});
+      uploads.push_back({bd.tab.type, bd.type.data(), (size_t)n * 4});
+    }
     if (rc == CLW_OK) rc = balloc(bd.groups.size() * sizeof(GroupHost), (void**)&bd.groups_d);
     if (rc == CLW_OK) rc = balloc((size_t)n * 4, (void**)&bd.tp_group_d);
-    if (rc == CLW_OK) rc = clw_h2d(ctx, bd.tp_group_d, bd.tp_group.data(), (size_t)n * 4, 1);
+    if (rc == CLW_OK) uploads.push_back({bd.tp_group_d, bd.tp_group.data(), (size_t)n * 4});
     if (rc == CLW_OK) rc = balloc(tiles * 4, (void**)&bd.item_gt_d);
     if (rc == CLW_OK) rc = balloc(sizeof(PlanHost), (void**)&bd.plan_d);
     if (rc == CLW_OK) rc = balloc((size_t)n * 8, (void**)&bd.from_d);
-    if (rc == CLW_OK) rc = clw_h2d(ctx, bd.from_d, bd.from.data(), (size_t)n * 8, 1);
+    if (rc == CLW_OK) uploads.push_back({bd.from_d, bd.from.data(), (size_t)n * 8});
     if (bd.mfma && rc == CLW_OK) {
       for (uint32_t gi = 0; gi < bd.groups.size(); ++gi) {
         const GroupHost& g = bd.groups[gi];
@@ -430,13 +511,20 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       std::copy(bd.mt_first.begin(), bd.mt_first.end(), words.begin() + nm);
       std::copy(bd.mt_n.begin(), bd.mt_n.end(), words.begin() + 2 * (size_t)nm);
       rc = balloc(words.size() * 4, (void**)&bd.mt_d);
-      if (rc == CLW_OK) rc = clw_h2d(ctx, bd.mt_d, words.data(), words.size() * 4, 1);
+      if (rc == CLW_OK) {
+        upload_words.push_back(std::move(words));
+        uploads.push_back({bd.mt_d, upload_words.back().data(), upload_words.back().size() * 4});
+      }
       // row slices of k_mfma_runs (16 B of partial per slice and test prime)
       bd.nslices = B->opt.mfma_slices ? B->opt.mfma_slices : kMfmaSlices;
       if (rc == CLW_OK) rc = balloc((size_t)bd.nslices * n * 8, (void**)&bd.pm_lo);
       if (rc == CLW_OK) rc = balloc((size_t)bd.nslices * n * 8, (void**)&bd.pm_hi);
       bd.pm_slices = bd.nslices;
     }
+    // a window's uploads: groups, tiles, active matrix-core tiles and their
+    // count, the plan twice, the count again
+    stage_bytes += 2 * (bd.groups.size() * sizeof(GroupHost) + tiles * 4 + ((size_t)bd.n_mt() + 1) * 4 +
+                        3 * sizeof(PlanHost) + 8 * 16);
     if (B->opt.external_scratch) {  // chunk products in a caller's buffer (clw_batch_set_scratch)
       bd.part_cap = 0;
       bd.part_own = false;
@@ -445,7 +533,35 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
       if (rc == CLW_OK) rc = balloc((size_t)bd.part_cap * 8, (void**)&bd.part_hi);
     }
   }
-  if (rc == CLW_OK) rc = clw_finish(ctx);
+  if (rc == CLW_OK && clwi_host_alloc(stage_bytes, (void**)&B->stage) == CLW_OK) {
+    B->stage_cap = stage_bytes;
+    B->stage_off = kStageBack;
+  } else if (rc == CLW_OK) {  // without it, uploads and counters take blocking copies
+    B->stage = nullptr;
+  }
+  // The uploads through one pinned block, then the Montgomery setup of each
+  // band (it reads p and type); ready_ev follows. Nothing here waits for the
+  // device: a request loop builds batches on one stream while others run,
+  // and clw_batch_bind makes the batch's next stream wait for ready_ev.
+  size_t up_total = 0;
+  for (const Upload& u : uploads) up_total += (u.bytes + 15) & ~(size_t)15;
+  if (rc == CLW_OK && clwi_host_alloc(up_total, (void**)&B->create_stage) != CLW_OK) B->create_stage = nullptr;
+  size_t off = 0;
+  for (const Upload& u : uploads) {
+    if (rc != CLW_OK) break;
+    if (B->create_stage) {
+      memcpy(B->create_stage + off, u.src, u.bytes);
+      rc = clw_hip(hipMemcpyAsync(u.dst, B->create_stage + off, u.bytes, hipMemcpyHostToDevice, clw_stream(ctx)),
+                   "table upload");
+      off += (u.bytes + 15) & ~(size_t)15;
+    } else {
+      rc = clw_h2d(ctx, u.dst, u.src, u.bytes, 1);
+    }
+  }
+  for (Band& bd : B->bands)
+    if (bd.L && rc == CLW_OK) rc = clw_setup(ctx, &bd.tab, 0);
+  if (rc == CLW_OK) rc = clw_hip(hipEventCreateWithFlags(&B->ready_ev, hipEventDisableTiming), "hipEventCreate");
+  if (rc == CLW_OK) rc = clw_hip(hipEventRecord(B->ready_ev, clw_stream(ctx)), "hipEventRecord");
   if (rc != CLW_OK) {
     delete B;
     return rc;
@@ -460,7 +576,9 @@ extern "C" int clw_batch_bind(clw_batch* B, clw_ctx* ctx) {
   if (!B || !ctx) return clw_fail(CLW_ERR_ARG, "clw_batch_bind: null argument");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_bind with a window in flight");
   if (ctx->device != B->ctx->device) return clw_fail(CLW_ERR_ARG, "clw_batch_bind: another device");
-  CK(clw_finish(B->ctx));
+  if (!B->quiesce(false)) return clw_fail(CLW_ERR_HIP, "clw_batch_bind: waiting for the batch's work failed");
+  // creation's uploads and setup: the new stream waits for them on the device
+  if (B->ready_ev) CK(clw_hip(hipStreamWaitEvent(clw_stream(ctx), B->ready_ev, 0), "hipStreamWaitEvent"));
   B->ctx = ctx;
   return CLW_OK;
 }
@@ -561,8 +679,8 @@ static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
     bd.plan.items_per_chunk = ipc;
     bd.plan.K = K;
     bd.plan.mfma = 0;  // set by mfma_window once the segment span is known
-    CK(clw_h2d(B->ctx, bd.groups_d, bd.groups.data(), bd.groups.size() * sizeof(GroupHost), 0));
-    if (ipc) CK(clw_h2d(B->ctx, bd.item_gt_d, bd.item_gt.data(), (size_t)ipc * 4, 0));
+    CK(stage_h2d(B, bd.groups_d, bd.groups.data(), bd.groups.size() * sizeof(GroupHost)));
+    if (ipc) CK(stage_h2d(B, bd.item_gt_d, bd.item_gt.data(), (size_t)ipc * 4));
     if (bd.mfma) {  // the tiles of the active groups, then their count
       const uint32_t nm = bd.n_mt();
       bd.mt_active.clear();
@@ -570,10 +688,10 @@ static int plan_window(clw_batch* B, uint64_t q0, uint64_t* active_total) {
         if (bd.groups[bd.mt_group[mt]].T >= q0) bd.mt_active.push_back(mt);
       bd.mt_active.push_back((uint32_t)bd.mt_active.size());
       const size_t na = bd.mt_active.size() - 1;
-      if (na) CK(clw_h2d(B->ctx, bd.mt_d + 3 * (size_t)nm, bd.mt_active.data(), na * 4, 0));
-      CK(clw_h2d(B->ctx, bd.mt_d + 4 * (size_t)nm, &bd.mt_active[na], 4, 0));
+      if (na) CK(stage_h2d(B, bd.mt_d + 3 * (size_t)nm, bd.mt_active.data(), na * 4));
+      CK(stage_h2d(B, bd.mt_d + 4 * (size_t)nm, &bd.mt_active[na], 4));
     }
-    CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 1));
+    CK(stage_h2d(B, bd.plan_d, &bd.plan, sizeof(PlanHost)));
     act += bd.active_tp;
   }
   *active_total = act;
@@ -616,8 +734,8 @@ static int mfma_plan(clw_batch* B, Band& bd, uint32_t slices) {
   bd.plan.mfma = slices;
   const uint32_t nm = bd.n_mt(), na = slices ? (uint32_t)bd.mt_active.size() - 1 : 0u;
   if (slices) CK(mfma_tables(B, bd));
-  CK(clw_h2d(B->ctx, bd.plan_d, &bd.plan, sizeof(PlanHost), 0));
-  CK(clw_h2d(B->ctx, bd.mt_d + 4 * (size_t)nm, &na, 4, 1));
+  CK(stage_h2d(B, bd.plan_d, &bd.plan, sizeof(PlanHost)));
+  CK(stage_h2d(B, bd.mt_d + 4 * (size_t)nm, &na, 4));
   return CLW_OK;
 }
 
@@ -700,6 +818,11 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   for (const Band& bd : B->bands)
     if (bd.L && bd.part_cap < bd.tab.count)
       return clw_fail(CLW_ERR_STATE, "batch without chunk-product slots (clw_batch_set_scratch)");
+  CK(stage_reset(B));
+  if (B->create_stage && hipEventQuery(B->ready_ev) == hipSuccess) {  // creation's uploads are done
+    clwi_host_release(B->create_stage);
+    B->create_stage = nullptr;
+  }
   uint64_t active = 0;
   CK(plan_window(B, qlo, &active));
   uint64_t span_blocks = B->max_blocks - 2;
@@ -727,7 +850,7 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   h.hi = lo;  // the first advance yields [lo, ...)
   h.span = span;
   h.end = std::max(qhi, lo);
-  CK(clw_h2d(ctx, B->seg_d, &h, sizeof(h), 1));
+  CK(stage_h2d(B, B->seg_d, &h, sizeof(h)));
   B->ev_used = 0;
   const bool timed = !B->opt.graph;
   for (size_t s = 0; s < segs.size(); ++s) {
@@ -765,6 +888,15 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   }
   // folds the last segment's prime count into the window total
   CK(clwi_seg_advance(ctx, B->seg_d, B->flags, B->totals + 1, B->totals));
+  // flags and counters into pinned memory, read by clw_batch_wait without a
+  // copy of its own
+  B->back_staged = false;
+  if (B->stage) {
+    CK(clw_hip(hipMemcpyAsync(B->stage, B->flags, 16, hipMemcpyDeviceToHost, clw_stream(ctx)), "counters copy"));
+    CK(clw_hip(hipMemcpyAsync(B->stage + 16, B->totals, kTotalsBytes, hipMemcpyDeviceToHost, clw_stream(ctx)),
+               "counters copy"));
+    B->back_staged = true;
+  }
   B->pending.active = true;
   if (!B->done_ev) {
     const hipError_t e = hipEventCreateWithFlags(&B->done_ev, hipEventDisableTiming);
@@ -800,8 +932,15 @@ extern "C" int clw_batch_wait(clw_batch* B) {
   const uint64_t qlo = B->pending.qlo, qhi = B->pending.qhi;
   uint32_t flags[4] = {0, 0, 0, 0};
   uint64_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  CK(clw_d2h(ctx, flags, B->flags, 16, 0));
-  CK(clw_d2h(ctx, tot, B->totals, sizeof(tot), 1));
+  static_assert(sizeof(tot) == kTotalsBytes && 16 + kTotalsBytes <= kStageBack, "counters in the staging area");
+  if (B->back_staged) {  // copied behind the window
+    CK(clw_hip(hipEventSynchronize(B->done_ev), "hipEventSynchronize"));
+    memcpy(flags, B->stage, 16);
+    memcpy(tot, B->stage + 16, sizeof(tot));
+  } else {
+    CK(clw_d2h(ctx, flags, B->flags, 16, 0));
+    CK(clw_d2h(ctx, tot, B->totals, sizeof(tot), 1));
+  }
   if (flags[0]) return clw_fail(CLW_ERR_OVERFLOW, "device capacity exceeded: flags %u", flags[0]);
   uint64_t smallc = 0;
   for (uint64_t q : {2ull, 3ull, 5ull}) smallc += (q >= qlo && q < qhi);
@@ -848,33 +987,75 @@ static int range_results(const clw_batch* B, uint32_t r, clw_range_stats* st, ui
 
 // One range done early: once the sieving primes up to its largest target
 // have been applied, iterate and check its test primes alone (the rest of
-// the batch keeps running) and return its results.
+// the batch keeps running). finish_launch enqueues that on the batch's
+// stream with a copy of the results into pinned host memory and returns;
+// finish_query tells whether they are in; finish_range collects them
+// (launching first if nothing is in flight) and returns the range's results.
+// The device buffer goes back to the idle-block cache, never to hipFree,
+// which would wait for every stream of the device.
+extern "C" int clw_batch_finish_launch(clw_batch* B, uint32_t r) {
+  if (!B || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_finish_launch: bad argument");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_launch with a window in flight");
+  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_launch after clw_batch_iterate");
+  RangeHost& R = B->ranges[r];
+  if (R.finished || R.finishing) return CLW_OK;
+  const uint64_t mt = std::max(R.T[0], std::max(R.T[1], R.T[2]));
+  if (B->next_q <= mt)
+    return clw_fail(CLW_ERR_STATE, "range %u needs sieving primes up to %llu, covered up to %llu", r,
+                    (unsigned long long)mt, (unsigned long long)B->next_q);
+  Band& bd = B->bands[R.band];
+  const uint32_t n = (uint32_t)R.tp.size();
+  const clw_tp_table v = table_view(bd.tab, R.band_off, n);
+  clw_ctx* ctx = B->ctx;
+  int rc = clw_alloc(ctx, (size_t)2 * n * 8, (void**)&R.fin_d);
+  if (rc == CLW_OK) rc = clwi_host_alloc((size_t)2 * n * 8, (void**)&R.fin_h);
+  if (rc == CLW_OK) rc = clw_iterate_from(ctx, &v, bd.from_d + R.band_off);
+  if (rc == CLW_OK) rc = clw_finalize(ctx, &v, R.fin_d, R.fin_d + n, nullptr);
+  if (rc == CLW_OK)
+    rc = clw_hip(hipMemcpyAsync(R.fin_h, R.fin_d, (size_t)2 * n * 8, hipMemcpyDeviceToHost, clw_stream(ctx)),
+                 "finish copy");
+  if (rc == CLW_OK && !R.fin_ev) rc = clw_hip(hipEventCreateWithFlags(&R.fin_ev, hipEventDisableTiming), "hipEventCreate");
+  if (rc == CLW_OK) rc = clw_hip(hipEventRecord(R.fin_ev, clw_stream(ctx)), "hipEventRecord");
+  if (rc != CLW_OK) {
+    // nothing may still use the buffers when they go back
+    (void)hipStreamSynchronize(clw_stream(ctx));
+    (void)clwi_release(ctx, R.fin_d);
+    clwi_host_release(R.fin_h);
+    R.fin_d = nullptr;
+    R.fin_h = nullptr;
+    return rc;
+  }
+  R.finishing = true;
+  return CLW_OK;
+}
+
+extern "C" int clw_batch_finish_query(const clw_batch* B, uint32_t r) {
+  if (!B || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_finish_query: bad argument");
+  const RangeHost& R = B->ranges[r];
+  if (R.finished) return 1;
+  if (!R.finishing) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_query: range %u not launched", r);
+  const hipError_t e = hipEventQuery(R.fin_ev);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return clw_fail_hip(e, "hipEventQuery");
+}
+
 extern "C" int clw_batch_finish_range(clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p,
                                       uint64_t* out_quot, uint64_t* out_rem, uint64_t cap) {
   if (!B || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_finish_range: bad argument");
-  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_range with a window in flight");
   RangeHost& R = B->ranges[r];
-  if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_finish_range after clw_batch_iterate");
   if (!R.finished) {
-    const uint64_t mt = std::max(R.T[0], std::max(R.T[1], R.T[2]));
-    if (B->next_q <= mt)
-      return clw_fail(CLW_ERR_STATE, "range %u needs sieving primes up to %llu (covered below %llu)", r,
-                      (unsigned long long)mt, (unsigned long long)B->next_q);
-    Band& bd = B->bands[R.band];
-    const uint32_t n = (uint32_t)R.tp.size();
-    const clw_tp_table v = table_view(bd.tab, R.band_off, n);
-    uint64_t *dq = nullptr, *dr = nullptr;
-    int rc = clw_alloc(B->ctx, (size_t)n * 8, (void**)&dq);
-    if (rc == CLW_OK) rc = clw_alloc(B->ctx, (size_t)n * 8, (void**)&dr);
-    if (rc == CLW_OK) rc = clw_iterate_from(B->ctx, &v, bd.from_d + R.band_off);
-    if (rc == CLW_OK) rc = clw_finalize(B->ctx, &v, dq, dr, nullptr);
-    R.quot.assign(n, 0);
-    R.rem.assign(n, 0);
-    if (rc == CLW_OK) rc = clw_d2h(B->ctx, R.quot.data(), dq, (size_t)n * 8, 0);
-    if (rc == CLW_OK) rc = clw_d2h(B->ctx, R.rem.data(), dr, (size_t)n * 8, 1);
-    clw_free(B->ctx, dq);
-    clw_free(B->ctx, dr);
-    CK(rc);
+    if (!R.finishing) CK(clw_batch_finish_launch(B, r));
+    CK(clw_hip(hipEventSynchronize(R.fin_ev), "hipEventSynchronize"));
+    const size_t n = R.tp.size();
+    R.quot.assign(R.fin_h, R.fin_h + n);
+    R.rem.assign(R.fin_h + n, R.fin_h + 2 * n);
+    // the copy, the buffers' last use, is behind the event
+    (void)clwi_release(B->ctx, R.fin_d);
+    clwi_host_release(R.fin_h);
+    R.fin_d = nullptr;
+    R.fin_h = nullptr;
+    R.finishing = false;
     R.finished = true;
   }
   return range_results(B, r, st, out_p, out_quot, out_rem, cap);
@@ -884,13 +1065,13 @@ extern "C" int clw_batch_iterate(clw_batch* B) {
   if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_iterate: null batch");
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_iterate called twice");
   bool any_finished = false;
-  for (const RangeHost& R : B->ranges) any_finished |= R.finished;
+  for (const RangeHost& R : B->ranges) any_finished |= R.finished || R.finishing;
   if (!any_finished) {
     for (Band& bd : B->bands)
       if (bd.L) CK(clw_iterate_from(B->ctx, &bd.tab, bd.from_d));
   } else {
     for (const RangeHost& R : B->ranges) {
-      if (R.finished) continue;
+      if (R.finished || R.finishing) continue;
       Band& bd = B->bands[R.band];
       const clw_tp_table v = table_view(bd.tab, R.band_off, (uint32_t)R.tp.size());
       CK(clw_iterate_from(B->ctx, &v, bd.from_d + R.band_off));

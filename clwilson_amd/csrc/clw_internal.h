@@ -87,6 +87,10 @@ int clwi_mfma_rows(clw_ctx* ctx, void* seg, uint32_t extra_block, const uint32_t
 // Returns a block from clw_alloc that no kernel uses any more (every stream
 // that used it has drained) to the device's cache of idle blocks.
 int clwi_release(clw_ctx* ctx, void* dptr);
+// Pinned host memory from a cache of idle blocks (clw_runtime.hip); a block
+// goes back with clwi_host_release once no copy uses it.
+int clwi_host_alloc(size_t bytes, void** hptr);
+void clwi_host_release(void* hptr);
 int clwi_mfma_runs(clw_ctx* ctx, const clw_tp_table* t, const void* groups, const uint32_t* mt_group,
                    const uint32_t* mt_first, const uint32_t* mt_n, const uint32_t* mt_active, const uint32_t* n_active,
                    uint32_t n_mt, const uint32_t* rowlist, const uint32_t* mrows, const void* subq,

@@ -193,6 +193,53 @@ extern "C" int clwi_release(clw_ctx* ctx, void* dptr) {
   return clw_free(ctx, dptr);
 }
 
+// Pinned host blocks for copies that must not block the host (a copy from
+// or to pageable memory does), kept for reuse by size class: hipHostFree,
+// like hipFree, may wait for the device. Blocks come back through
+// clwi_host_release once no copy uses them; at most kHostCacheCap bytes are
+// kept idle.
+namespace {
+constexpr size_t kHostCacheCap = 1ull << 30;
+std::unordered_multimap<size_t, void*> g_host_cache;  // size class -> idle pinned block
+std::unordered_map<void*, size_t> g_host_size;        // every pinned block handed out
+size_t g_host_cached = 0;
+}  // namespace
+
+extern "C" int clwi_host_alloc(size_t bytes, void** hptr) {
+  if (!hptr) return clw_fail(CLW_ERR_ARG, "clwi_host_alloc: null argument");
+  const size_t cls = size_class(bytes == 0 ? 16 : bytes);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_host_cache.find(cls);
+  if (it != g_host_cache.end()) {
+    *hptr = it->second;
+    g_host_cache.erase(it);
+    g_host_cached -= cls;
+    return CLW_OK;
+  }
+  const hipError_t e = hipHostMalloc(hptr, cls, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *hptr = nullptr;
+    clw_fail_hip(e, "hipHostMalloc");
+    return CLW_ERR_NOMEM;
+  }
+  g_host_size[*hptr] = cls;
+  return CLW_OK;
+}
+
+extern "C" void clwi_host_release(void* hptr) {
+  if (!hptr) return;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_host_size.find(hptr);
+  if (it == g_host_size.end()) return;
+  if (g_host_cached + it->second <= kHostCacheCap) {
+    g_host_cache.emplace(it->second, hptr);
+    g_host_cached += it->second;
+    return;
+  }
+  g_host_size.erase(it);
+  (void)hipHostFree(hptr);
+}
+
 extern "C" int clw_free(clw_ctx* ctx, void* dptr) {
   if (!dptr) return CLW_OK;
   if (ctx) CK_HIP_OR_RETURN(hipSetDevice(ctx->device), "hipSetDevice");

@@ -22,8 +22,11 @@ stream and are served together:
 * Scheduling. Shortest work to the next completion first: the cohorts whose
   next range needs the fewest factor applications to finish go first, and a
   window stops at that range's largest factorial target, so a range finishes
-  (clw_batch_finish_range: iterate + check of its primes alone) as soon as
-  its sieving primes are in, while larger ranges of the same cohort go on.
+  (clw_batch_finish_launch: iterate + check of its primes alone, queued on
+  the cohort's stream) as soon as its sieving primes are in, while larger
+  ranges of the same cohort go on. Its results are collected when they are
+  in (clw_batch_finish_query), so the loop never waits for a finish while
+  other streams run out of work.
 * Latency is completion time minus arrival time; each request's results are
   those of a single-range run, bit for bit.
 """
@@ -66,7 +69,8 @@ class _Cohort:
     T: np.ndarray                        # group targets (one group per range and type)
     n: np.ndarray                        # test primes per group
     admitted: float
-    done: set = field(default_factory=set)
+    done: set = field(default_factory=set)        # ranges finished and collected
+    finishing: set = field(default_factory=set)   # ranges whose finish is in flight
     bytes: int = 0
     rate: float = 0.0                    # factor applications/s it has been running at
     window: tuple = ()                   # (lo, hi, launch time, counter) of the window in flight
@@ -138,6 +142,7 @@ class Server:
         self.reserved = 0                    # estimated bytes of cohorts being built
         self.peak_used = 0
         self.completed: list[Completion] = []
+        self.finishing: list[tuple[_Cohort, int]] = []  # finishes in flight (launched, not collected)
         self.factor_apps = 0
         self.windows = 0
         self.admit_seconds = 0.0
@@ -318,21 +323,37 @@ class Server:
         dt = time.perf_counter() - t
         if lo >= (1 << 20) and dt > 0 and apps > 0:
             c.rate = 0.5 * c.rate + 0.5 * apps / dt
-        done = 0
+        launched = 0
         cov = c.covered()
         for r, m in enumerate(c.maxt):
-            if r not in c.done and m < cov:
-                res = c.batch.finish_range(r)
-                c.done.add(r)
-                self.completed.append(Completion(c.reqs[r].rid, c.reqs[r].arrival, c.admitted, self.now(), res))
-                done += 1
-        if len(c.done) == len(c.reqs):
-            c.batch.close()
-            with self.lock:
-                self.used -= c.bytes
-            self.cohorts.remove(c)
-        else:
-            c.work = c.work_to_next()
+            if r not in c.done and r not in c.finishing and m < cov:
+                c.batch.finish_launch(r)  # queued behind the window on the cohort's stream
+                c.finishing.add(r)
+                self.finishing.append((c, r))
+                launched += 1
+        c.work = c.work_to_next()
+        return launched
+
+    def _collect(self, block: bool = False) -> int:
+        """Results of the finishes that are in (all of them with block);
+        a cohort whose ranges are all collected is retired."""
+        done = 0
+        keep = []
+        for c, r in self.finishing:
+            if not block and not c.batch.finish_ready(r):
+                keep.append((c, r))
+                continue
+            res = c.batch.finish_range(r)
+            c.finishing.discard(r)
+            c.done.add(r)
+            self.completed.append(Completion(c.reqs[r].rid, c.reqs[r].arrival, c.admitted, self.now(), res))
+            done += 1
+            if len(c.done) == len(c.reqs):
+                c.batch.close()
+                with self.lock:
+                    self.used -= c.bytes
+                self.cohorts.remove(c)
+        self.finishing = keep
         return done
 
     def poll(self) -> bool:
@@ -341,7 +362,7 @@ class Server:
         anything happened."""
         if self._error is not None:
             raise self._error
-        busy = False
+        busy = self._collect() > 0
         if self.ready:
             self._bind_ready()
             busy = True
@@ -369,6 +390,7 @@ class Server:
         if self._thread is not None:
             raise RuntimeError("stop admission before a snapshot")
         self.drain()
+        self._collect(block=True)
         self._bind_ready()
         os.makedirs(directory, exist_ok=True)
         cohorts = []
@@ -408,7 +430,7 @@ class Server:
 
     def idle(self) -> bool:
         with self.lock:
-            return not (self.queue or self.ready or self.cohorts or self.reserved)
+            return not (self.queue or self.ready or self.cohorts or self.reserved or self.finishing)
 
     def serve(self, requests: list[Request], duration: float, progress=None, until_done: bool = False) -> dict:
         """Replays `requests` in real time (arrival seconds after start) for

@@ -379,6 +379,16 @@ uint64_t clw_batch_device_bytes(const clw_batch* b);
  * returns its results as clw_batch_results does. */
 int clw_batch_finish_range(clw_batch* b, uint32_t r, clw_range_stats* stats, uint64_t* out_p,
                            uint64_t* out_quot, uint64_t* out_rem, uint64_t out_cap);
+/* clw_batch_finish_range in two halves, for a request loop that must not
+ * block (no counterpart in the reference, which finishes one range per
+ * process): finish_launch enqueues range r's iterate and check on the batch's
+ * stream, with a copy of its results into pinned host memory, and returns
+ * (no window may be in flight; windows may be launched after it);
+ * finish_query returns 1 once the results are in, 0 while they are not
+ * (never blocks; CLW_ERR_STATE if range r was not launched);
+ * clw_batch_finish_range then collects them. */
+int clw_batch_finish_launch(clw_batch* b, uint32_t r);
+int clw_batch_finish_query(const clw_batch* b, uint32_t r);
 /* Sieving primes applied so far (all windows). */
 uint64_t clw_batch_applied(const clw_batch* b);
 /* Canonical running residues of range r, in increasing p. */

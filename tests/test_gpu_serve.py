@@ -77,6 +77,40 @@ def test_finish_range_early_matches_full_batch(dev):
         _same(b.results(2), ref[2])
 
 
+def test_finish_launch_collect_matches_full_batch(dev):
+    """Finishing in two halves: the range's iterate and check queued on the
+    stream (finish_launch), a window of the other ranges launched behind it,
+    then the results collected once finish_ready says so; they equal the
+    whole batch's. A finish of a range whose sieving primes are not all in is
+    refused, and so is a query of a range never launched."""
+    ranges = [(1_000_000, 1_002_000), (400_000_000, 400_002_000), (50_000, 60_000)]
+    with cw.Batch(dev, ranges) as b:
+        ref = b.search()
+    with cw.Batch(dev, ranges) as b:
+        maxt = [st.maxtarget for st in b.info]
+        b.run(2, maxt[2] + 1)
+        with pytest.raises(cw.ClwError):
+            b.finish_ready(2)  # not launched
+        b.finish_launch(2)
+        b.finish_launch(2)  # a second launch of the same range does nothing
+        with pytest.raises(cw.ClwError):
+            b.finish_launch(1)  # its sieving primes are not all in yet
+        b.launch(maxt[2] + 1, maxt[0] + 1)  # behind the finish on the same stream
+        b.wait()
+        assert b.finish_ready(2)
+        early = b.finish_range(2)
+        assert b.finish_ready(2)  # collected: stays ready
+        b.finish_launch(0)
+        b.run(maxt[0] + 1, b.maxtarget + 1)
+        b.iterate()  # skips ranges 0 and 2
+        b.finalize()
+        early0 = b.finish_range(0)
+        _same(early, ref[2])
+        _same(early0, ref[0])
+        _same(b.results(1), ref[1])
+        _same(b.results(2), ref[2])
+
+
 def test_server_snapshot_restore(dev, tmp_path):
     """A server stopped part-way and snapshotted, then restored into a new
     server, completes every request with the single-range results."""
