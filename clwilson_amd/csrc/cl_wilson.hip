@@ -81,10 +81,16 @@ int64_t signed_w(uint64_t p, uint64_t quot) {
 
 }  // namespace
 
-// Segmented sieve over [lo, hi) with base primes up to min(sqrt(hi), 2^25)
-// (complete for hi below 1.1e15); above, survivors are confirmed by
-// deterministic Miller-Rabin. Replaces primesieve_generate_primes
-// (cl_wilson.cpp:1529).
+// Segmented sieve over [lo, hi) of the odd numbers, then (when it stops
+// short of sqrt(hi)) deterministic Miller-Rabin on what is left. Replaces
+// primesieve_generate_primes (cl_wilson.cpp:1529). How far to sieve: a base
+// prime q costs one 64-bit division (its first multiple in the range) and
+// takes out about 1/q of the candidates, and a candidate left costs a
+// Miller-Rabin test (about 50 divisions' worth); a narrow range far out
+// (1e4 wide at 1e14) is best sieved by the primes below 2^16 and tested, a
+// wide one sieved to sqrt(hi) (base primes up to 2^25 are kept: complete
+// below 1.1e15). Each base prime's next multiple is carried from window to
+// window, so its division is paid once per range.
 extern "C" uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uint64_t cap) {
   if (hi <= lo) return 0;
   uint64_t cnt = 0;
@@ -93,33 +99,58 @@ extern "C" uint64_t clw_host_primes(uint64_t lo, uint64_t hi, uint64_t* out, uin
     ++cnt;
   };
   if (lo <= 2 && hi > 2) emit(2);
-  uint64_t s = lo < 3 ? 3 : (lo | 1);
+  const uint64_t s = lo < 3 ? 3 : (lo | 1);
   if (s >= hi) return cnt;
-  uint64_t r = (uint64_t)clw::isqrt64(hi) + 1;
-  const uint32_t lim = (uint32_t)std::min<uint64_t>(r, 1u << 25);
-  const bool need_mr = r > lim;
-  const auto base_ptr = base_primes(lim);
-  const std::vector<uint32_t>& base = *base_ptr;
-  const uint64_t SEG = 1u << 20;  // odd numbers per window
-  std::vector<uint8_t> mark(SEG);
-  for (uint64_t a = s; a < hi; a += 2 * SEG) {
-    const uint64_t b = std::min<uint64_t>(a + 2 * SEG, hi);
-    const uint64_t n = (b - a + 1) / 2;
-    std::fill(mark.begin(), mark.begin() + n, 0);
-    for (uint32_t q32 : base) {
-      const uint64_t q = q32;
-      if (q * q >= b) break;
-      uint64_t st = q * q;
-      if (st < a) {
-        st = (a + q - 1) / q * q;
-        if ((st & 1) == 0) st += q;
+  const uint64_t nodd = (hi - s + 1) / 2;  // candidates s, s + 2, ..., below hi
+  const uint64_t r = (uint64_t)clw::isqrt64(hi - 1);  // a composite below hi has a prime factor <= r
+  const uint64_t top = std::min<uint64_t>(r, 1u << 25);  // the furthest the kept base primes reach
+  uint64_t lim = top;
+  {
+    double best = -1;
+    for (uint64_t q = 1u << 16;; q <<= 2) {
+      const uint64_t qq = std::min(q, top);
+      const double lq = std::log((double)std::max<uint64_t>(qq, 16));
+      double cost = (double)qq / lq;                          // divisions
+      if (qq < r) cost += 50.0 * (double)nodd * 1.1229 / lq;  // Miller-Rabin on the survivors
+      if (best < 0 || cost < best) {
+        best = cost;
+        lim = qq;
       }
-      for (uint64_t m = st; m < b; m += 2 * q) mark[(m - a) >> 1] = 1;
+      if (qq == top) break;
     }
-    for (uint64_t i = 0; i < n; ++i) {
-      const uint64_t v = a + 2 * i;
-      if (v >= b) break;
-      if (mark[i] || v < 3) continue;
+  }
+  const bool need_mr = lim < r;
+  const auto base_ptr = base_primes((uint32_t)lim);
+  const std::vector<uint32_t>& base = *base_ptr;
+  size_t nb = 0;  // base primes in use: q <= lim
+  while (nb < base.size() && base[nb] <= lim) ++nb;
+  // next[j]: odd-candidate index of base[j]'s next multiple to strike (the
+  // first at or above max(q^2, s))
+  std::vector<uint64_t> next(nb);
+  for (size_t j = 0; j < nb; ++j) {
+    const uint64_t q = base[j];
+    uint64_t st = q * q;
+    if (st < s) {
+      st = (s + q - 1) / q * q;
+      if ((st & 1) == 0) st += q;
+    }
+    next[j] = (st - s) / 2;
+  }
+  const uint64_t SEG = 1u << 18;  // odd candidates per window (256 KB of flags)
+  std::vector<uint8_t> mark(SEG);
+  for (uint64_t w0 = 0; w0 < nodd; w0 += SEG) {
+    const uint64_t w1 = std::min(w0 + SEG, nodd);
+    std::fill(mark.begin(), mark.begin() + (w1 - w0), 0);
+    for (size_t j = 0; j < nb; ++j) {
+      uint64_t k = next[j];
+      if (k >= w1) continue;
+      const uint64_t q = base[j];
+      for (; k < w1; k += q) mark[k - w0] = 1;
+      next[j] = k;
+    }
+    for (uint64_t k = w0; k < w1; ++k) {
+      if (mark[k - w0]) continue;
+      const uint64_t v = s + 2 * k;
       if (need_mr && !is_prime_det(v)) continue;
       emit(v);
     }

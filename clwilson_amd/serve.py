@@ -112,10 +112,10 @@ class Server:
     gives a process by default, so no stream waits behind another's kernel
     in a shared queue."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.2,
-                 apps_per_second: float = 1e13, segment_blocks: int = 640, max_cohort: int = 64,
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.5,
+                 apps_per_second: float = 1e13, segment_blocks: int = 2560, max_cohort: int = 64,
                  admit_interval: float = 0.5, host_primes: bool = True, class_bits: int = 4,
-                 cohort_primes: int = 0):
+                 cohort_primes: int = 0, run_products: int = 2):
         self.devices = [Device(device) for _ in range(streams)]
         self.admit_dev = Device(device)
         # chunk products of every cohort on a stream live in that stream's
@@ -124,15 +124,18 @@ class Server:
         self.budget = hbm_budget
         self.step_seconds = step_seconds
         self.rate = apps_per_second
-        # segments of 640 blocks (39 M integers, ~1.4 M sieving primes at 1e12):
-        # long enough for the 3-limb band's run products to go to the matrix
-        # cores (about a thousand sub-chunks per slice)
+        # segments of 2560 blocks (157 M integers, ~5.7 M sieving primes at
+        # 1e12): a launch of the matrix-core run products then has some 5,500
+        # sub-chunks per slice to spread its per-workgroup costs (table, fold)
+        # over; at 640 blocks a replay sustained 5.5e12 factor applications/s,
+        # at 2560 8.6e12, and 9.7e12 with windows of 0.5 s instead of 0.2 s
         self.segment_span = segment_blocks * BLOCK_SPAN
         self.host_primes = host_primes
         self.max_cohort = max_cohort
         self.class_bits = class_bits          # magnitude classes 2^class_bits wide
         self.cohort_primes = cohort_primes or self.COHORT_PRIMES  # estimated test primes per cohort at most
         self.admit_interval = admit_interval
+        self.run_products = run_products      # clw_options.run_products of every cohort
         self.lock = threading.Lock()
         self.queue: list[Request] = []
         self.ready: list[_Cohort] = []       # built, not yet bound to a stream
@@ -245,7 +248,7 @@ class Server:
         b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
                   segment_apps=int(self.rate * self.step_seconds), host_primes=self.host_primes,
                   external_scratch=True,
-                  run_products=2)
+                  run_products=self.run_products)
         groups = [(st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t]]
         c = _Cohort(b, -1, take, [st.maxtarget for st in b.info], np.array([g[0] for g in groups], dtype=np.float64),
                     np.array([g[1] for g in groups], dtype=np.int64), self.now())

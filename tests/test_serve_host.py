@@ -65,7 +65,9 @@ def make_server(monkeypatch, **kw):
 
 
 def test_admission_respects_budget_and_classes(monkeypatch):
-    srv = make_server(monkeypatch, hbm_budget=3 * S.Server.SCRATCH_SLOTS * 16 + (200 << 20), streams=3)
+    # 640-block segments: about 170 MB of window buffers per cohort
+    srv = make_server(monkeypatch, hbm_budget=3 * S.Server.SCRATCH_SLOTS * 16 + (200 << 20), streams=3,
+                      segment_blocks=640)
     reqs = [Request(i, 0.0, 10**9 * (1 + i % 7) * (1000 if i % 2 else 1), 10**9 * (1 + i % 7) *
                     (1000 if i % 2 else 1) + 2_000_000) for i in range(40)]
     for r in reqs:
