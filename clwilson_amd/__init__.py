@@ -106,7 +106,8 @@ class Options(C.Structure):
     _fields_ = [("segment_span", C.c_uint64), ("target_waves", C.c_uint32), ("verbose", C.c_uint32),
                 ("mode", C.c_uint32), ("per_lane", C.c_uint32), ("graph", C.c_uint32),
                 ("segment_apps", C.c_uint64), ("host_primes", C.c_uint32), ("max_chunks", C.c_uint32),
-                ("external_scratch", C.c_uint32), ("run_products", C.c_uint32), ("mfma_slices", C.c_uint32)]
+                ("external_scratch", C.c_uint32), ("run_products", C.c_uint32), ("mfma_slices", C.c_uint32),
+                ("external_workspace", C.c_uint32)]
 
 
 class RangeStats(C.Structure):
@@ -208,6 +209,10 @@ def lib():
         "clw_batch_device_bytes": (u64, [vp]),
         "clw_batch_finish_range": (i32, [vp, u32, P(RangeStats), vp, vp, vp, u64]),
         "clw_batch_finish_launch": (i32, [vp, u32]),
+        "clw_workspace_create": (i32, [vp, u64, P(vp)]),
+        "clw_workspace_destroy": (i32, [vp]),
+        "clw_workspace_bytes": (u64, [vp]),
+        "clw_batch_set_workspace": (i32, [vp, vp]),
         "clw_batch_finish_query": (i32, [vp, u32]),
     }
     for name, (res, args) in sig.items():
@@ -481,6 +486,23 @@ class Scratch:
             self.lo = self.hi = None
 
 
+class Workspace:
+    """Window buffers (record buffer, sieve bitmap, run products) shared by
+    batches that run their windows one after another on one stream
+    (clw_workspace_*), sized for batches of the given segment span."""
+
+    def __init__(self, dev: "Device", segment_span: int = 0):
+        self.dev = dev
+        self.h = C.c_void_p()
+        check(lib().clw_workspace_create(dev.ctx, segment_span, C.byref(self.h)), "workspace_create")
+        self.bytes = lib().clw_workspace_bytes(self.h)
+
+    def close(self) -> None:
+        if self.h:
+            lib().clw_workspace_destroy(self.h)
+            self.h = C.c_void_p()
+
+
 class Batch:
     """Many ranges searched together over one shared sieve (clw_batch_*).
     Each range's results equal clw_wilson_range's on it, bit for bit."""
@@ -488,7 +510,7 @@ class Batch:
     def __init__(self, dev: "Device", ranges: list[tuple[int, int]], segment_span: int = 0,
                  target_waves: int = 0, per_lane: int = 0, graph: bool = True, segment_apps: int = 0,
                  host_primes: bool = False, max_chunks: int = 0, external_scratch: bool = False,
-                 run_products: int = 0, mfma_slices: int = 0):
+                 run_products: int = 0, mfma_slices: int = 0, external_workspace: bool = False):
         if not ranges:
             raise ValueError("a batch needs at least one range")
         self.dev = dev
@@ -498,7 +520,7 @@ class Batch:
         hi = np.array([b for _, b in self.ranges], dtype=np.uint64)
         opt = Options(segment_span, target_waves, 0, 0, per_lane, 1 if graph else 0, segment_apps,
                       1 if host_primes else 0, max_chunks, 1 if external_scratch else 0,
-                      run_products, mfma_slices)
+                      run_products, mfma_slices, 1 if external_workspace else 0)
         check(lib().clw_batch_create(dev.ctx, lo.ctypes.data, hi.ctypes.data, len(self.ranges), C.byref(opt),
                                      C.byref(self.h)), "batch_create")
         self.info = []
@@ -547,6 +569,11 @@ class Batch:
     def set_scratch(self, scratch: "Scratch") -> None:
         """Keep chunk products in a shared device scratch (see Scratch)."""
         check(lib().clw_batch_set_scratch(self.h, scratch.lo, scratch.hi, scratch.cap), "batch_set_scratch")
+
+    def set_workspace(self, ws: Workspace) -> None:
+        """Run this batch's windows in ws's buffers (ws shared by batches
+        whose windows run one after another on one stream)."""
+        check(lib().clw_batch_set_workspace(self.h, ws.h), "batch_set_workspace")
 
     def bind(self, dev: "Device") -> None:
         """Run this batch on another device context (stream) from now on."""

@@ -148,6 +148,20 @@ int64_t signed_w(uint64_t p, uint64_t quot) {
 
 }  // namespace
 
+// Window buffers that batches running their windows on one stream can share
+// (clw_workspace_create): record buffer and counts, sieve bitmap, sub-chunk
+// digits and bytes, row lists; sized for any batch of the same segment span
+// (digits for sieving primes up to 2^62).
+struct clw_workspace {
+  clw_ctx* ctx = nullptr;
+  uint32_t max_blocks = 0;
+  uint32_t *rec = nullptr, *nrec = nullptr, *subp = nullptr, *subn = nullptr, *era = nullptr;
+  int8_t* subb = nullptr;
+  void* subq = nullptr;
+  uint32_t *subm = nullptr, *rowlist = nullptr, *rowpos = nullptr, *mrows = nullptr;
+  uint64_t bytes = 0;
+};
+
 struct clw_batch {
   clw_ctx* ctx = nullptr;
   clw_options opt{};
@@ -171,6 +185,9 @@ struct clw_batch {
   uint32_t* rowpos = nullptr;   // and each sub-chunk's place in that list
   uint32_t* era = nullptr;      // wheel bitmap of the sieve of Eratosthenes (segments below 2^50)
   uint32_t* mrows = nullptr;
+  // the window buffers above belong to a clw_workspace (clw_batch_set_workspace)
+  // and are not the batch's to free
+  bool ws_borrowed = false;
   bool any_mfma = false;
   SegHost* seg_d = nullptr;
   hipGraph_t graph = nullptr;
@@ -263,8 +280,12 @@ struct clw_batch {
         give(b.part_hi);
       }
     }
-    void* ptrs[] = {rec, nrec, flags, totals, seg_d, subp, subn, subb, subq, subm, rowlist, rowpos, mrows, era};
-    for (void* x : ptrs) give(x);
+    void* own[] = {flags, totals, seg_d};
+    for (void* x : own) give(x);
+    if (!ws_borrowed) {
+      void* win[] = {rec, nrec, subp, subn, subb, subq, subm, rowlist, rowpos, mrows, era};
+      for (void* x : win) give(x);
+    }
   }
 };
 
@@ -401,15 +422,19 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     B->bytes += sz;
     return clw_alloc(ctx, sz, ptr);
   };
-  int rc = balloc((size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
+  // window buffers: the batch's own, or (external_workspace) a shared
+  // workspace's, given by clw_batch_set_workspace before the first window
+  const bool own_win = !B->opt.external_workspace;
+  int rc = CLW_OK;
+  if (own_win) rc = balloc((size_t)B->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&B->rec);
   // every segment ends at most at maxT + 1: below 2^50 the segments are
   // sieved by Eratosthenes (the table of sieving primes built here, before
   // any graph capture), else by trial division and Miller-Rabin
   if (rc == CLW_OK && B->maxT + 1 <= clwi_era_hi_max()) {
-    rc = balloc((size_t)B->max_blocks * (CLW_BLOCK_SPAN / 30), (void**)&B->era);
+    if (own_win) rc = balloc((size_t)B->max_blocks * (CLW_BLOCK_SPAN / 30), (void**)&B->era);
     if (rc == CLW_OK) rc = clwi_era_prepare(ctx, B->maxT + 1);
   }
-  if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * 4, (void**)&B->nrec);
+  if (rc == CLW_OK && own_win) rc = balloc((size_t)B->max_blocks * 4, (void**)&B->nrec);
   if (rc == CLW_OK) rc = balloc(16, (void**)&B->flags);
   if (rc == CLW_OK) rc = balloc(kTotalsBytes, (void**)&B->totals);
   if (rc == CLW_OK) rc = balloc(sizeof(SegHost), (void**)&B->seg_d);
@@ -420,8 +445,8 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     const uint32_t digits = (2 * kBatchSubRecs * bits + LB_DIGIT - 1) / LB_DIGIT;
     B->ds = std::min<uint32_t>(kBatchSubMaxDigits, std::max<uint32_t>(12, (digits + 11) / 12 * 12));
   }
-  if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);
-  if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subn);
+  if (rc == CLW_OK && own_win) rc = balloc((size_t)B->max_blocks * kBatchSubCap * B->ds * 4, (void**)&B->subp);
+  if (rc == CLW_OK && own_win) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subn);
   if (B->opt.run_products > 2) {
     const uint32_t v = B->opt.run_products;
     delete B;
@@ -449,7 +474,7 @@ extern "C" int clw_batch_create(clw_ctx* ctx, const uint64_t* pmin, const uint64
     bd.mfma_lazy = L == 4 || 2.0 * (double)pmax * (double)pmax < 0.9996 * std::ldexp(1.0, 90);
     B->any_mfma = true;
   }
-  if (B->any_mfma) {
+  if (B->any_mfma && own_win) {
     if (rc == CLW_OK) rc = balloc(subb_bytes(B), (void**)&B->subb);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 16, &B->subq);
     if (rc == CLW_OK) rc = balloc((size_t)B->max_blocks * kBatchSubCap * 4, (void**)&B->subm);
@@ -585,6 +610,85 @@ extern "C" int clw_batch_bind(clw_batch* B, clw_ctx* ctx) {
 
 extern "C" int clw_batch_destroy(clw_batch* B) {
   delete B;
+  return CLW_OK;
+}
+
+extern "C" int clw_workspace_create(clw_ctx* ctx, uint64_t segment_span, clw_workspace** out) {
+  if (!ctx || !out) return clw_fail(CLW_ERR_ARG, "clw_workspace_create: null argument");
+  *out = nullptr;
+  clw_workspace* W = new clw_workspace();
+  W->ctx = ctx;
+  const uint32_t span_blocks =
+      segment_span ? (uint32_t)std::min<uint64_t>((segment_span + CLW_BLOCK_SPAN - 1) / CLW_BLOCK_SPAN, kMaxSpanBlocks)
+                   : kMaxSpanBlocks;
+  W->max_blocks = std::max<uint32_t>(kMinSpanBlocks, span_blocks) + 2;  // as clw_batch_create sizes them
+  const size_t slots = (size_t)W->max_blocks * kBatchSubCap;
+  auto walloc = [&](size_t sz, void** ptr) {
+    W->bytes += sz;
+    return clw_alloc(ctx, sz, ptr);
+  };
+  int rc = walloc((size_t)W->max_blocks * CLW_REC_CAP * CLW_REC_WORDS * 4, (void**)&W->rec);
+  if (rc == CLW_OK) rc = walloc((size_t)W->max_blocks * (CLW_BLOCK_SPAN / 30), (void**)&W->era);
+  if (rc == CLW_OK) rc = walloc((size_t)W->max_blocks * 4, (void**)&W->nrec);
+  if (rc == CLW_OK) rc = walloc(slots * kBatchSubMaxDigits * 4, (void**)&W->subp);
+  if (rc == CLW_OK) rc = walloc(slots * 4, (void**)&W->subn);
+  if (rc == CLW_OK) rc = walloc(mfma_rows_bytes(slots), (void**)&W->subb);
+  if (rc == CLW_OK) rc = walloc(slots * 16, &W->subq);
+  if (rc == CLW_OK) rc = walloc(slots * 4, (void**)&W->subm);
+  if (rc == CLW_OK) rc = walloc(slots * 4, (void**)&W->rowlist);
+  if (rc == CLW_OK) rc = walloc(slots * 4, (void**)&W->rowpos);
+  if (rc == CLW_OK) rc = walloc(4, (void**)&W->mrows);
+  if (rc != CLW_OK) {
+    clw_workspace_destroy(W);
+    return rc;
+  }
+  *out = W;
+  return CLW_OK;
+}
+
+// The caller has destroyed (or rebound) every batch that used the workspace.
+extern "C" int clw_workspace_destroy(clw_workspace* W) {
+  if (!W) return CLW_OK;
+  const bool drained = hipStreamSynchronize(clw_stream(W->ctx)) == hipSuccess;
+  void* ptrs[] = {W->rec, W->era, W->nrec, W->subp, W->subn, W->subb, W->subq, W->subm, W->rowlist, W->rowpos, W->mrows};
+  for (void* x : ptrs) (void)(drained ? clwi_release(W->ctx, x) : clw_free(W->ctx, x));
+  delete W;
+  return CLW_OK;
+}
+
+extern "C" uint64_t clw_workspace_bytes(const clw_workspace* W) { return W ? W->bytes : 0; }
+
+// The batch's windows use W's buffers from now on (its own, if it had any,
+// go back to the cache once its work is done).
+extern "C" int clw_batch_set_workspace(clw_batch* B, clw_workspace* W) {
+  if (!B || !W) return clw_fail(CLW_ERR_ARG, "clw_batch_set_workspace: null argument");
+  if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_set_workspace with a window in flight");
+  if (W->ctx->device != B->ctx->device) return clw_fail(CLW_ERR_ARG, "clw_batch_set_workspace: another device");
+  if (W->max_blocks < B->max_blocks)
+    return clw_fail(CLW_ERR_ARG, "workspace of %u blocks < the batch's %u", W->max_blocks, B->max_blocks);
+  if (!B->ws_borrowed && B->rec) {  // the batch's own buffers
+    if (!B->quiesce()) return clw_fail(CLW_ERR_HIP, "clw_batch_set_workspace: waiting for the batch's work failed");
+    void* win[] = {B->rec, B->nrec, B->subp, B->subn, B->subb, B->subq, B->subm, B->rowlist, B->rowpos, B->mrows, B->era};
+    for (void* x : win) (void)clwi_release(B->ctx, x);
+  }
+  B->rec = W->rec;
+  B->nrec = W->nrec;
+  B->subp = W->subp;
+  B->subn = W->subn;
+  B->era = B->maxT + 1 <= clwi_era_hi_max() ? W->era : nullptr;
+  B->subb = B->any_mfma ? W->subb : nullptr;
+  B->subq = W->subq;
+  B->subm = W->subm;
+  B->rowlist = W->rowlist;
+  B->rowpos = W->rowpos;
+  B->mrows = W->mrows;
+  B->ws_borrowed = true;
+  if (B->graph_exec) {  // captured with the old buffers
+    CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
+    CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
+    B->graph_exec = nullptr;
+    B->graph = nullptr;
+  }
   return CLW_OK;
 }
 
@@ -809,6 +913,7 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_launch: null batch");
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_launch after iterate");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_launch: previous window not waited for");
+  if (!B->rec) return clw_fail(CLW_ERR_STATE, "batch without window buffers (clw_batch_set_workspace)");
   clw_ctx* ctx = B->ctx;
   const auto t0 = std::chrono::steady_clock::now();
   if (qhi > B->maxT + 1) qhi = B->maxT + 1;
@@ -1430,6 +1535,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
   if (!B) return clw_fail(CLW_ERR_ARG, "clw_batch_autotune: null batch");
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune after iterate");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune with a window in flight");
+  if (!B->rec) return clw_fail(CLW_ERR_STATE, "batch without window buffers (clw_batch_set_workspace)");
   clw_ctx* ctx = B->ctx;
   if (q < 7) q = 7;
   if (q > B->maxT) q = B->maxT;

@@ -37,11 +37,12 @@ import math
 import os
 import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Batch, Device, RangeResult, Scratch, host_primes
+from . import Batch, Device, RangeResult, Scratch, Workspace, host_primes
 from .workloads import Request, pi_approx
 
 BLOCK_SPAN = 61440
@@ -115,7 +116,8 @@ class Server:
     def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, streams: int = 3, step_seconds: float = 0.5,
                  apps_per_second: float = 1e13, segment_blocks: int = 2560, max_cohort: int = 64,
                  admit_interval: float = 0.5, host_primes: bool = True, class_bits: int = 4,
-                 cohort_primes: int = 0, run_products: int = 2):
+                 cohort_primes: int = 0, run_products: int = 2, admit_wait: float = 0.0,
+                 min_cohort_primes: int = 1 << 20, max_cohorts: int = 48):
         self.devices = [Device(device) for _ in range(streams)]
         self.admit_dev = Device(device)
         # chunk products of every cohort on a stream live in that stream's
@@ -130,18 +132,31 @@ class Server:
         # over; at 640 blocks a replay sustained 5.5e12 factor applications/s,
         # at 2560 8.6e12, and 9.7e12 with windows of 0.5 s instead of 0.2 s
         self.segment_span = segment_blocks * BLOCK_SPAN
+        # and the window buffers (record buffer, sieve bitmap, run products)
+        # of every cohort on a stream in that stream's workspace: a cohort
+        # then holds only its tables and accumulators
+        self.workspaces = [Workspace(d, self.segment_span) for d in self.devices]
         self.host_primes = host_primes
         self.max_cohort = max_cohort
         self.class_bits = class_bits          # magnitude classes 2^class_bits wide
         self.cohort_primes = cohort_primes or self.COHORT_PRIMES  # estimated test primes per cohort at most
         self.admit_interval = admit_interval
         self.run_products = run_products      # clw_options.run_products of every cohort
+        self.admit_wait = admit_wait          # seconds a class may wait to grow into a cohort
+        self.min_cohort_primes = min_cohort_primes
+        # cohorts built and not retired at most (0: no cap). Fewer, larger
+        # cohorts sieve each segment for more test primes: a 30 s replay ran
+        # at 9.1e12 factor applications/s with 48 (75 requests completed,
+        # p50 8.8 s), 7.3e12 with 96 (104, 7.3 s), 6.5e12 with no cap (152,
+        # 4.3 s)
+        self.max_cohorts = max_cohorts
         self.lock = threading.Lock()
         self.queue: list[Request] = []
         self.ready: list[_Cohort] = []       # built, not yet bound to a stream
         self.cohorts: list[_Cohort] = []
         self.running: list[_Cohort | None] = [None] * streams
-        self.used = len(self.devices) * self.SCRATCH_SLOTS * 16  # scratch + built cohorts (device bytes)
+        # scratch + workspaces + built cohorts (device bytes)
+        self.used = len(self.devices) * self.SCRATCH_SLOTS * 16 + sum(w.bytes for w in self.workspaces)
         self.reserved = 0                    # estimated bytes of cohorts being built
         self.peak_used = 0
         self.completed: list[Completion] = []
@@ -158,6 +173,7 @@ class Server:
     SCRATCH_SLOTS = 1 << 23    # chunk-product slots per stream (2 x 64 MiB)
     COHORT_PRIMES = 1 << 22    # estimated test primes per cohort at most
     FILL_PRIMES = 1 << 18      # test primes that fill the GPU in one window
+    BUILD_THREADS = 4          # cohorts built at once in an admission round
 
     def close(self) -> None:
         self.stop_admission()
@@ -167,6 +183,8 @@ class Server:
         self.cohorts, self.ready = [], []
         for sc in self.scratch:
             sc.close()
+        for w in self.workspaces:
+            w.close()
         for d in self.devices + [self.admit_dev]:
             d.close()
 
@@ -207,47 +225,80 @@ class Server:
 
     def admit(self) -> int:
         """One admission round; returns the number of requests admitted."""
-        # record buffer (4096 records of 32 B per block) and run products
-        # (64 sub-chunks of at most 276 digits + a count per block; for the
-        # matrix cores also 768 bytes, 24 B of row data and a row index)
-        fixed = (self.segment_span // BLOCK_SPAN + 2) * (4096 * 32 + 64 * 277 * 4 + 64 * 796) + (1 << 20)
+        # a cohort's own buffers beside its tables (descriptor, counters,
+        # plans); its window buffers are its stream's workspace
+        fixed = 1 << 20
         classes: dict[int, list[Request]] = {}
         est: dict[int, int] = {}
         primes: dict[int, float] = {}
         with self.lock:
-            while self.queue:
-                r = self.queue[0]
+            # at most max_cohorts cohorts at a time: requests queue meanwhile
+            # and the next round makes fewer, larger cohorts of them
+            room = self.max_cohorts - len(self.cohorts) - len(self.ready) if self.max_cohorts else 1 << 30
+            if room <= 0:
+                return 0
+            # a class waits until its queued requests make a cohort of
+            # min_cohort_primes test primes or its oldest has waited
+            # admit_wait seconds: small cohorts each sieve every segment for
+            # few test primes
+            now = self.now()
+            total: dict[int, float] = {}
+            oldest: dict[int, float] = {}
+            for r in self.queue:
                 k = self.magnitude_class(r, self.class_bits)
+                total[k] = total.get(k, 0.0) + self.estimate_primes(r)
+                oldest.setdefault(k, r.arrival)
+            ready = {k for k in total if total[k] >= self.min_cohort_primes or now - oldest[k] >= self.admit_wait}
+            waiting: list[Request] = []
+            i = 0
+            while i < len(self.queue):
+                r = self.queue[i]
+                k = self.magnitude_class(r, self.class_bits)
+                if k not in ready:
+                    waiting.append(r)
+                    i += 1
+                    continue
                 grp = classes.get(k)
                 if grp is not None and (len(grp) >= self.max_cohort or
                                         primes[k] + self.estimate_primes(r) > self.cohort_primes):
                     break
+                if grp is None and len(classes) >= room:
+                    break
                 e = self.estimate_bytes(r) + (0 if grp else fixed)
                 if self.used + self.reserved + e > self.budget:
                     break
-                classes.setdefault(k, []).append(self.queue.pop(0))
+                classes.setdefault(k, []).append(r)
                 est[k] = est.get(k, 0) + e
                 primes[k] = primes.get(k, 0.0) + self.estimate_primes(r)
                 self.reserved += e
+                i += 1
+            self.queue = waiting + self.queue[i:]
         t = time.perf_counter()
-        n = 0
-        for k in sorted(classes):
-            take = classes[k]
-            c = self._build(take)
-            n += len(take)
+
+        def build(k):
+            c = self._build(classes[k])
             with self.lock:
                 self.ready.append(c)
                 self.used += c.bytes
                 self.reserved -= est[k]
                 self.peak_used = max(self.peak_used, self.used)
+
+        # the classes' cohorts are built side by side (the library releases
+        # the interpreter; their uploads share the admission stream)
+        ks = sorted(classes)
+        if len(ks) > 1:
+            with ThreadPoolExecutor(max_workers=min(len(ks), self.BUILD_THREADS)) as ex:
+                list(ex.map(build, ks))
+        elif ks:
+            build(ks[0])
         self.admit_seconds += time.perf_counter() - t
-        return n
+        return sum(len(classes[k]) for k in ks)
 
     def _build(self, take: list[Request]) -> _Cohort:
         """A cohort of requests: their batch built on the admission stream."""
         b = Batch(self.admit_dev, [(r.pmin, r.pmax) for r in take], segment_span=self.segment_span,
                   segment_apps=int(self.rate * self.step_seconds), host_primes=self.host_primes,
-                  external_scratch=True,
+                  external_scratch=True, external_workspace=True,
                   run_products=self.run_products)
         groups = [(st.typeTarget[t], st.tpcnt[t]) for st in b.info for t in range(3) if st.tpcnt[t]]
         c = _Cohort(b, -1, take, [st.maxtarget for st in b.info], np.array([g[0] for g in groups], dtype=np.float64),
@@ -302,6 +353,7 @@ class Server:
             c.slot = min(kinds, key=lambda s: load[s])
             c.batch.bind(self.devices[c.slot])
             c.batch.set_scratch(self.scratch[c.slot])
+            c.batch.set_workspace(self.workspaces[c.slot])
             self.cohorts.append(c)
 
     def _launch(self, slot: int) -> bool:

@@ -255,6 +255,9 @@ typedef struct {
                               (8, 16 or 32; 0 = 8); each slice's rows are read
                               by one XCD, each slice adds 16 B of partial per
                               test prime; the same results                    */
+  uint32_t external_workspace; /* batches: allocate no window buffers (record
+                              buffer, sieve bitmap, run products); the caller
+                              provides a workspace (clw_batch_set_workspace) */
 } clw_options;
 
 typedef struct {
@@ -356,6 +359,18 @@ int clw_batch_bind(clw_batch* b, clw_ctx* ctx);
  * (at least the batch's test primes): batches that run one after another on
  * one stream can share one scratch instead of holding their own. */
 int clw_batch_set_scratch(clw_batch* b, uint64_t* part_lo, uint64_t* part_hi, uint64_t cap);
+/* Window buffers shared by batches that run their windows one after another
+ * on one stream (no counterpart in the reference, which runs one range per
+ * process; for a request loop holding many batches): the record buffer, the
+ * sieve bitmap and the run products of a segment, sized for any batch of
+ * the given segment span (0 = the default). A batch created with
+ * external_workspace has none of its own until clw_batch_set_workspace; a
+ * batch with its own gives them back. The workspace outlives its batches. */
+typedef struct clw_workspace clw_workspace;
+int clw_workspace_create(clw_ctx* ctx, uint64_t segment_span, clw_workspace** out);
+int clw_workspace_destroy(clw_workspace* ws);
+uint64_t clw_workspace_bytes(const clw_workspace* ws);
+int clw_batch_set_workspace(clw_batch* b, clw_workspace* ws);
 uint32_t clw_batch_size(const clw_batch* b);
 /* Counts and targets of range r (as clw_session_info). */
 int clw_batch_info(const clw_batch* b, uint32_t r, clw_range_stats* stats);

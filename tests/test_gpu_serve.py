@@ -38,8 +38,8 @@ def test_serving_alone_vs_mix(dev, streams):
     about three cohorts at a time: every result equals the request run alone
     (the session path), and the budget is never exceeded."""
     reqs = [Request(i, 0.025 * i, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
-    budget = streams * Server.SCRATCH_SLOTS * 16 + (30 << 20)  # per-stream scratch + ~3 cohorts
-    srv = Server(hbm_budget=budget, streams=streams, step_seconds=0.01, admit_interval=0.05, segment_blocks=64)
+    srv = Server(hbm_budget=1 << 40, streams=streams, step_seconds=0.01, admit_interval=0.05, segment_blocks=64)
+    budget = srv.budget = srv.used + (30 << 20)  # per-stream scratch and workspace + ~3 cohorts
     try:
         rep = srv.serve(reqs, duration=100.0, until_done=True)
     finally:
@@ -111,11 +111,46 @@ def test_finish_launch_collect_matches_full_batch(dev):
         _same(b.results(2), ref[2])
 
 
+def test_batches_sharing_a_workspace(dev):
+    """Two batches whose windows alternate on one stream in one workspace's
+    buffers (record buffer, sieve bitmap, run products on the matrix cores at
+    ~2e9) give the results of the same batches with buffers of their own; a
+    batch created without window buffers refuses a window until it has a
+    workspace."""
+    ra = [(2_000_000_000, 2_000_004_000), (1_000_000_000, 1_000_002_000)]
+    rb = [(3_000_000_000, 3_000_003_000), (50_000, 60_000)]
+    with cw.Batch(dev, ra) as a, cw.Batch(dev, rb) as b:
+        ref_a, ref_b = a.search(), b.search()
+    ws = cw.Workspace(dev)
+    try:
+        with cw.Batch(dev, ra, external_workspace=True) as a, cw.Batch(dev, rb, external_workspace=True) as b:
+            with pytest.raises(cw.ClwError):
+                a.run(2, 1000)
+            a.set_workspace(ws)
+            b.set_workspace(ws)
+            cuts = [2, 10**8, 4 * 10**8]
+            for x in (a, b):
+                x.cuts = cuts + [x.maxtarget + 1]
+            for k in range(len(cuts)):
+                for x in (a, b):
+                    say("window", k, x.cuts[k], x.cuts[k + 1])
+                    x.run(x.cuts[k], x.cuts[k + 1])
+            assert a.kernel_stats()["mfma_int8_macs"] > 0
+            for x in (a, b):
+                x.iterate()
+                x.finalize()
+            for r in range(2):
+                _same(a.results(r), ref_a[r])
+                _same(b.results(r), ref_b[r])
+    finally:
+        ws.close()
+
+
 def test_server_snapshot_restore(dev, tmp_path):
     """A server stopped part-way and snapshotted, then restored into a new
     server, completes every request with the single-range results."""
     reqs = [Request(i, 0.0, lo, hi) for i, (lo, hi) in enumerate(RANGES)]
-    srv = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.002, segment_blocks=64)
+    srv = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.002, segment_blocks=64, admit_wait=0.0)
     try:
         for r in reqs:
             srv.submit(r)
@@ -127,7 +162,7 @@ def test_server_snapshot_restore(dev, tmp_path):
     finally:
         srv.close()
     say("completed before the snapshot:", sorted(before))
-    srv2 = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.01, segment_blocks=64)
+    srv2 = Server(hbm_budget=1 << 30, streams=2, step_seconds=0.01, segment_blocks=64, admit_wait=0.0)
     try:
         done = srv2.restore(str(tmp_path))
         assert sorted(done) == sorted(before)
