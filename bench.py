@@ -61,8 +61,8 @@ SPAN = 4370 * 61440  # integers per step (one default segment)
 # kernel's source file when the counters were taken: if the file has changed
 # since, the figure belongs to another kernel and the line reports null.
 MFMA_SOURCE = os.path.join(ROOT, "clwilson_amd", "csrc", "clw_mfma_kernels.inc")
-MFMA_TRAFFIC = dict(fetch_kib=30654747.2, write_kib=572724.0, fetch_scale=262144 / 131088.25, write_scale=1.0,
-                    source_sha256="708c03286b519d808a0b5a0631a4672b57c1714cdc45132966df058c493729d4",
+MFMA_TRAFFIC = dict(fetch_kib=30703995.2, write_kib=572724.0, fetch_scale=262144 / 131088.0, write_scale=1.0,
+                    source_sha256="073f8db48098ebd68dbbf429f92a3c9a513e34461c64ecd62ab398431e616d4f",
                     window="one segment of 268,492,800 sieving integers just below T0 = 3.33e12")
 # The reference formulation's declared cost per sieving-prime factor
 # application at p ~ 2e13 in 32-bit partial products (mul_r2 22 + square 22.9
