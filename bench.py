@@ -562,6 +562,9 @@ def main() -> int:
         "launches": ks["mul_launches"],
         "mean_launch_ms": vec_ms / max(1, ks["mul_launches"]),
     }
+    if not ks["mul_partial_products"]:
+        vec_line["note"] = ("no product on the vector units in these windows: every sub-chunk lies inside one "
+                            "exponent run and goes to the matrix cores, so the kernel only walks its record list")
     if ks["mfma_int8_macs"]:
         mfma_launches = max(1, ks["sieve_launches"])
         ops = 2.0 * ks["mfma_int8_macs"] / (ks["mfma_ms"] * 1e-3) if ks["mfma_ms"] else 0.0

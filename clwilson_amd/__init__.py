@@ -81,7 +81,7 @@ class BandTune(C.Structure):
                 ("trial_items_per_block", C.c_uint32 * 16), ("trial_apps_per_s", C.c_double * 16),
                 ("mfma_slices", C.c_uint32), ("mfma_min_rows", C.c_uint32), ("mfma_trials", C.c_uint32),
                 ("mfma_trial_slices", C.c_uint32 * 8), ("mfma_trial_rows", C.c_uint64 * 8),
-                ("mfma_trial_ms", C.c_double * 8)]
+                ("mfma_trial_ms", C.c_double * 8), ("trial_ms", C.c_double * 16)]
 
     def as_dict(self) -> dict:
         n = min(self.trials, 16)
@@ -91,6 +91,7 @@ class BandTune(C.Structure):
                     active_test_primes=self.active_test_primes, best_apps_per_s=self.best_apps_per_s,
                     candidates=[(self.trial_per_lane[k], self.trial_items_per_block[k], self.trial_apps_per_s[k])
                                 for k in range(n)],
+                    candidate_ms=[self.trial_ms[k] for k in range(n)],
                     mfma_slices=self.mfma_slices, mfma_min_rows=self.mfma_min_rows,
                     mfma_candidates=[(self.mfma_trial_slices[k], self.mfma_trial_rows[k], self.mfma_trial_ms[k])
                                      for k in range(m)])

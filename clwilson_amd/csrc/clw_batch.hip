@@ -1599,6 +1599,7 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
             t.trial_per_lane[t.trials] = a;
             t.trial_items_per_block[t.trials] = b;
             t.trial_apps_per_s[t.trials] = rate;
+            t.trial_ms[t.trials] = ms;
           }
           t.trials++;
           if (ms < best_ms) {

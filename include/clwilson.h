@@ -441,7 +441,8 @@ int clw_batch_results(const clw_batch* b, uint32_t r, clw_range_stats* stats, ui
  *     take the run products, the records across run boundaries only:
  *     best_apps_per_s and trial_apps_per_s are then that kernel's own
  *     factor applications per second, 0 for a segment with no record
- *     across a run boundary); the fastest on the trial segment is kept.
+ *     across a run boundary); the fastest on the trial segment is kept
+ *     (trial_ms: each candidate's time).
  * With cache_path, a tuned (device, band, size class) is read from /
  * appended to that text file instead of re-timed. report receives one entry
  * per band present (at most report_cap). The reference sizes its launches
@@ -459,6 +460,7 @@ typedef struct {
   uint32_t mfma_trial_slices[8];
   uint64_t mfma_trial_rows[8];
   double mfma_trial_ms[8];
+  double trial_ms[16];    /* step 2: each per-lane trial's time on the trial segment (the fastest is kept) */
 } clw_band_tune;
 int clw_batch_autotune(clw_batch* b, uint64_t q, const char* cache_path, clw_band_tune* report,
                        uint32_t report_cap);

@@ -239,7 +239,31 @@ def cfg4_windows():
     dump("cfg4_windows.json", dict(windows=CFG4_WINDOWS, ranges=rows))
 
 
+LARGE_Q = ((20_000_000_000_000, 20_000_000_100_000), (10**12, 10**12 + 2 * 10**9))
+
+
+def large_q_window():
+    """Oracle window products for sampled test primes of [2e13, 2e13+1e5)
+    over the sieving primes of [1e12, 1e12+2e9), all above sqrt(T): the
+    window of the batch's run-product carry test."""
+    (lo, hi), (qlo, qhi) = LARGE_Q
+    allp = O.primes_range(lo, hi)
+    typ, T = type_targets(allp)
+    n = len(allp)
+    idx = sorted(set([0, 1, 2, n // 4, n // 2, 3 * n // 4, n - 2, n - 1]))
+    ps = np.array([allp[i] for i in idx], dtype=np.uint64)
+    targets = np.array([T[typ[i]] for i in idx], dtype=np.uint64)
+    t0 = time.time()
+    vals = O.window_product(ps, targets, qlo, qhi)
+    print("large-q window", round(time.time() - t0, 1), flush=True)
+    dump("large_q_window.json", dict(pmin=lo, pmax=hi, qlo=qlo, qhi=qhi, tpcount=n, index=idx,
+                                     p=[int(x) for x in ps], targets=[int(x) for x in targets],
+                                     value=[str(v) for v in vals]))
+
+
 def main():
+    if sys.argv[1:] == ["--large-q"]:
+        return large_q_window()
     if sys.argv[1:] == ["--direct"]:
         return direct_only()
     if sys.argv[1:] == ["--mfma"]:

@@ -96,3 +96,31 @@ def test_cli_help_and_argument_parsing():
     assert r.returncode == 2 and "invalid argument" in r.stdout
     r = subprocess.run([exe, "-p", "2"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "out of range" in r.stdout
+
+
+# ctypes mirror of each struct the header declares for the binding
+_STRUCTS = {"DeviceInfo": "clw_device_info", "TpTable": "clw_tp_table", "SegBuffers": "clw_seg_buffers",
+            "KernelStats": "clw_kernel_stats", "BandTune": "clw_band_tune", "WorkStatus": "clw_work_status",
+            "Options": "clw_options", "RangeStats": "clw_range_stats"}
+
+
+def test_binding_structs_match_header_layout(tmp_path):
+    """Every ctypes struct of the binding has the size and field offsets that
+    a C compiler gives the header's struct (a field added on one side only
+    shifts everything after it)."""
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "clwilson.h"', "int main(void) {"]
+    want = []
+    for py, c in _STRUCTS.items():
+        cls = getattr(cw, py)
+        lines.append(f'  printf("%zu\\n", sizeof({c}));')
+        want.append(C.sizeof(cls))
+        for name, *_ in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({c}, {name}));')
+            want.append(getattr(cls, name).offset)
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == want

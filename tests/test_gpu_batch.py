@@ -126,18 +126,23 @@ def test_batch_window_residues_all_bands(dev, window):
                 assert b.residues(r) == s.residues()
 
 
-def test_batch_run_products_carry_large_q(dev):
+def test_batch_run_products_carry_large_q(dev, golden):
     """Above sqrt(T) almost every sieving prime enters through a run product
     (one full Montgomery product per 128-prime sub-chunk, the digits reduced
     mod p^2), and the window residues equal the session path's, which applies
-    every record as a product."""
-    lo, hi = 2 * 10**13, 2 * 10**13 + 100_000
-    qlo, qhi = 10**12, 10**12 + 2 * 10**9
+    every record as a product, and, for sampled test primes, the oracle's
+    window products (mullarge.cl:36-64 restated)."""
+    g = golden("large_q_window.json")
+    lo, hi = g["pmin"], g["pmax"]
+    qlo, qhi = g["qlo"], g["qhi"]
     with cw.Batch(dev, [(lo, hi)]) as b, cw.Session(dev, lo, hi) as s:
         b.run(qlo, qhi)
         s.run(qlo, qhi)
         ks = b.kernel_stats()
-        assert b.residues(0) == s.residues()
+        res = b.residues(0)
+        assert b.info[0].tpcount == g["tpcount"]
+        assert [res[i] for i in g["index"]] == [int(v) for v in g["value"]]
+        assert res == s.residues()
     say("prime apps", ks["mul_prime_apps"], "full products", ks["mul_record_apps"],
         "partial products", ks["mul_partial_products"])
     assert ks["mul_prime_apps"] > 0
@@ -214,6 +219,7 @@ def test_autotune_keeps_results(dev, tmp_path):
         assert rep and all(t["trials"] >= 3 and not t["from_cache"] for t in rep)
         assert all(t["per_lane"] in (1, 2, 3, 4, 5, 6) for t in rep)
         assert all(rate > 0 for t in rep for _, _, rate in t["candidates"])  # every trial did its work
+        assert all(ms > 0 for t in rep for ms in t["candidate_ms"])
         assert [b.residues(r) for r in range(len(ranges))] == before
         b.run(1000, b.maxtarget + 1)
         b.iterate()
@@ -251,6 +257,7 @@ def test_autotune_matrix_cores_keeps_results(dev, tmp_path):
         assert len(cand) >= 4 and {s for s, _, _ in cand} >= {0, 8} and all(ms > 0 for _, _, ms in cand)
         # per-lane trials time the vector kernel alone on what the matrix cores leave it
         assert t["candidates"] and all(rate >= 0 for _, _, rate in t["candidates"])
+        assert len(t["candidate_ms"]) == len(t["candidates"]) and all(ms > 0 for ms in t["candidate_ms"])
         assert b.residues(0) == before
         b.run(*w)
         assert b.residues(0) == ref
