@@ -188,6 +188,7 @@ struct clw_batch {
   // the window buffers above belong to a clw_workspace (clw_batch_set_workspace)
   // and are not the batch's to free
   bool ws_borrowed = false;
+  const clw_workspace* ws = nullptr;
   bool any_mfma = false;
   SegHost* seg_d = nullptr;
   hipGraph_t graph = nullptr;
@@ -663,7 +664,9 @@ extern "C" uint64_t clw_workspace_bytes(const clw_workspace* W) { return W ? W->
 extern "C" int clw_batch_set_workspace(clw_batch* B, clw_workspace* W) {
   if (!B || !W) return clw_fail(CLW_ERR_ARG, "clw_batch_set_workspace: null argument");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_set_workspace with a window in flight");
-  if (W->ctx->device != B->ctx->device) return clw_fail(CLW_ERR_ARG, "clw_batch_set_workspace: another device");
+  // one stream orders the windows of every batch in the workspace's buffers
+  if (W->ctx != B->ctx)
+    return clw_fail(CLW_ERR_ARG, "clw_batch_set_workspace: the workspace belongs to another context (stream)");
   if (W->max_blocks < B->max_blocks)
     return clw_fail(CLW_ERR_ARG, "workspace of %u blocks < the batch's %u", W->max_blocks, B->max_blocks);
   if (!B->ws_borrowed && B->rec) {  // the batch's own buffers
@@ -683,6 +686,7 @@ extern "C" int clw_batch_set_workspace(clw_batch* B, clw_workspace* W) {
   B->rowpos = W->rowpos;
   B->mrows = W->mrows;
   B->ws_borrowed = true;
+  B->ws = W;
   if (B->graph_exec) {  // captured with the old buffers
     CK(clw_hip(hipGraphExecDestroy(B->graph_exec), "hipGraphExecDestroy"));
     CK(clw_hip(hipGraphDestroy(B->graph), "hipGraphDestroy"));
@@ -914,6 +918,8 @@ extern "C" int clw_batch_launch(clw_batch* B, uint64_t qlo, uint64_t qhi) {
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_launch after iterate");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_launch: previous window not waited for");
   if (!B->rec) return clw_fail(CLW_ERR_STATE, "batch without window buffers (clw_batch_set_workspace)");
+  if (B->ws && B->ws->ctx != B->ctx)
+    return clw_fail(CLW_ERR_STATE, "batch bound to another stream than its workspace (clw_batch_set_workspace again)");
   clw_ctx* ctx = B->ctx;
   const auto t0 = std::chrono::steady_clock::now();
   if (qhi > B->maxT + 1) qhi = B->maxT + 1;
@@ -1536,6 +1542,8 @@ extern "C" int clw_batch_autotune(clw_batch* B, uint64_t q, const char* cache_pa
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune after iterate");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_autotune with a window in flight");
   if (!B->rec) return clw_fail(CLW_ERR_STATE, "batch without window buffers (clw_batch_set_workspace)");
+  if (B->ws && B->ws->ctx != B->ctx)
+    return clw_fail(CLW_ERR_STATE, "batch bound to another stream than its workspace (clw_batch_set_workspace again)");
   clw_ctx* ctx = B->ctx;
   if (q < 7) q = 7;
   if (q > B->maxT) q = B->maxT;

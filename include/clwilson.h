@@ -365,7 +365,11 @@ int clw_batch_set_scratch(clw_batch* b, uint64_t* part_lo, uint64_t* part_hi, ui
  * sieve bitmap and the run products of a segment, sized for any batch of
  * the given segment span (0 = the default). A batch created with
  * external_workspace has none of its own until clw_batch_set_workspace; a
- * batch with its own gives them back. The workspace outlives its batches. */
+ * batch with its own gives them back. The batch must be bound to the
+ * workspace's context (its stream orders every window in the buffers):
+ * set_workspace refuses another context, and a batch rebound elsewhere
+ * afterwards refuses windows until it is given a workspace of its new
+ * context. The workspace outlives its batches. */
 typedef struct clw_workspace clw_workspace;
 int clw_workspace_create(clw_ctx* ctx, uint64_t segment_span, clw_workspace** out);
 int clw_workspace_destroy(clw_workspace* ws);

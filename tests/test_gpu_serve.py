@@ -126,6 +126,21 @@ def test_batches_sharing_a_workspace(dev):
         with cw.Batch(dev, ra, external_workspace=True) as a, cw.Batch(dev, rb, external_workspace=True) as b:
             with pytest.raises(cw.ClwError):
                 a.run(2, 1000)
+            other = cw.Device(0)  # another context (stream) on the same GPU
+            try:
+                ws_other = cw.Workspace(other)
+                try:
+                    with pytest.raises(cw.ClwError):
+                        a.set_workspace(ws_other)
+                finally:
+                    ws_other.close()
+                a.set_workspace(ws)
+                a.bind(other)  # now on another stream than its workspace
+                with pytest.raises(cw.ClwError):
+                    a.run(2, 1000)
+                a.bind(dev)
+            finally:
+                other.close()
             a.set_workspace(ws)
             b.set_workspace(ws)
             cuts = [2, 10**8, 4 * 10**8]
