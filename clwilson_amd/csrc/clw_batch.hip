@@ -1151,23 +1151,29 @@ extern "C" int clw_batch_finish_query(const clw_batch* B, uint32_t r) {
   return clw_fail_hip(e, "hipEventQuery");
 }
 
+// Waits for range r's launched finish and keeps its results on the host.
+static int collect_finish(clw_batch* B, RangeHost& R) {
+  CK(clw_hip(hipEventSynchronize(R.fin_ev), "hipEventSynchronize"));
+  const size_t n = R.tp.size();
+  R.quot.assign(R.fin_h, R.fin_h + n);
+  R.rem.assign(R.fin_h + n, R.fin_h + 2 * n);
+  // the copy, the buffers' last use, is behind the event
+  (void)clwi_release(B->ctx, R.fin_d);
+  clwi_host_release(R.fin_h);
+  R.fin_d = nullptr;
+  R.fin_h = nullptr;
+  R.finishing = false;
+  R.finished = true;
+  return CLW_OK;
+}
+
 extern "C" int clw_batch_finish_range(clw_batch* B, uint32_t r, clw_range_stats* st, uint64_t* out_p,
                                       uint64_t* out_quot, uint64_t* out_rem, uint64_t cap) {
   if (!B || r >= B->ranges.size()) return clw_fail(CLW_ERR_ARG, "clw_batch_finish_range: bad argument");
   RangeHost& R = B->ranges[r];
   if (!R.finished) {
     if (!R.finishing) CK(clw_batch_finish_launch(B, r));
-    CK(clw_hip(hipEventSynchronize(R.fin_ev), "hipEventSynchronize"));
-    const size_t n = R.tp.size();
-    R.quot.assign(R.fin_h, R.fin_h + n);
-    R.rem.assign(R.fin_h + n, R.fin_h + 2 * n);
-    // the copy, the buffers' last use, is behind the event
-    (void)clwi_release(B->ctx, R.fin_d);
-    clwi_host_release(R.fin_h);
-    R.fin_d = nullptr;
-    R.fin_h = nullptr;
-    R.finishing = false;
-    R.finished = true;
+    CK(collect_finish(B, R));
   }
   return range_results(B, r, st, out_p, out_quot, out_rem, cap);
 }
@@ -1682,6 +1688,10 @@ extern "C" int clw_batch_ckpt_write(clw_batch* B, const char* path) {
   if (!B || !path) return clw_fail(CLW_ERR_ARG, "clw_batch_ckpt_write: null argument");
   if (B->pending.active) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_write with a window in flight");
   if (B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_write after clw_batch_iterate");
+  // a launched finish has iterated its range's accumulators in place: the
+  // file takes that range's results, never those values as residues
+  for (RangeHost& R : B->ranges)
+    if (R.finishing) CK(collect_finish(B, R));
   std::vector<CkptRange> rr(B->ranges.size());
   std::vector<uint64_t> data;
   std::vector<uint64_t> acc_lo[6], acc_hi[6];

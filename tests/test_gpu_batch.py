@@ -297,6 +297,32 @@ def test_batch_host_prime_source(dev):
             _same(x, y)
 
 
+def test_batch_checkpoint_with_a_finish_in_flight(dev, tmp_path):
+    """A checkpoint written while a range's finish is launched but not yet
+    collected (its accumulators already iterated in place) keeps that range's
+    results, not those values as residues: resumed, every range ends with the
+    uninterrupted results."""
+    ranges = MIX[2:8]
+    with cw.Batch(dev, ranges) as b:
+        ref = b.search()
+    a, bp = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    with cw.Batch(dev, ranges) as b:
+        m2 = b.info[2].maxtarget
+        b.run(2, m2 + 1)
+        b.finish_launch(2)
+        b.checkpoint(a)  # collects the finish first
+        assert b.finish_ready(2)
+        _same(b.finish_range(2), ref[2])
+    with cw.Batch(dev, ranges) as b:
+        assert b.resume(a, bp) == (1, 0)
+        assert b.covered() == m2 + 1
+        b.run(b.covered(), b.maxtarget + 1)
+        b.iterate()
+        b.finalize()
+        for r in range(len(ranges)):
+            _same(b.results(r), ref[r])
+
+
 def test_batch_checkpoint_resume(dev, tmp_path):
     """A batch checkpointed part-way (A, then B), rebuilt and resumed from
     the newer valid file, ends with the uninterrupted results; a corrupt B
