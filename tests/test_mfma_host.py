@@ -141,3 +141,39 @@ def test_extreme_table_columns():
         assert worst < 2**31
     V = sum(sum(a[i] * tab[i][k] for i in range(KB)) << (8 * k) for k in range(COLS))
     assert V + (N << 17) > 0
+
+
+def _in_one_run(q_first, q_last, T, sqrtT):
+    """The batch kernels' test that a sub-chunk of sieving primes
+    [q_first, q_last] lies in one exponent run of target T above sqrt(T)
+    (then k_mfma_runs applies it and k_batch_mul skips it)."""
+    return q_first > sqrtT and q_last <= T and q_last <= T // (T // q_first)
+
+
+def test_whole_chunk_skip_implies_every_sub_chunk_taken():
+    """k_batch_mul skips a work item when its chunk's whole integer span
+    [qa, qz] passes the one-run test. Brute force over small targets: such a
+    span holds one exponent floor(T / q) (the one of the Legendre formula for
+    q > sqrt(T)), and every sub-interval [qf, ql] of it passes the per-block
+    test, so no sub-chunk is left to the vector units; a span one past the
+    run's end fails it."""
+    import math
+    rng = random.Random(5)
+    checked = 0
+    for T in [97, 1000, 4096, 12345, 99991] + [rng.randrange(50, 200000) for _ in range(30)]:
+        sqrtT = math.isqrt(T)
+        for _ in range(300):
+            qa = rng.randrange(2, T + 2)
+            qz = qa + rng.randrange(0, 2 * T // max(qa, 1) + 3)
+            if not _in_one_run(qa, qz, T, sqrtT):
+                continue
+            checked += 1
+            e = T // qa
+            assert all(T // q == e and q > sqrtT and q <= T for q in range(qa, qz + 1))
+            for _ in range(5):
+                qf = rng.randrange(qa, qz + 1)
+                ql = rng.randrange(qf, qz + 1)
+                assert _in_one_run(qf, ql, T, sqrtT)
+            # the run's last q is T // e: one past it starts the next run (or passes T)
+            assert not _in_one_run(qa, T // e + 1, T, sqrtT)
+    assert checked > 1000
