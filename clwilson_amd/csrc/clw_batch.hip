@@ -1770,6 +1770,9 @@ extern "C" int clw_batch_ckpt_read(clw_batch* B, const char* path_a, const char*
                                    int* result) {
   if (!B || !which || !result) return clw_fail(CLW_ERR_ARG, "clw_batch_ckpt_read: null argument");
   if (B->pending.active || B->iterated) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_read: batch in use");
+  // a launched finish would later hand back results of the state replaced here
+  for (const RangeHost& R : B->ranges)
+    if (R.finishing) return clw_fail(CLW_ERR_STATE, "clw_batch_ckpt_read: a range's finish is in flight");
   *which = -1;
   *result = 0;
   CkptHead ha{}, hb{};

@@ -474,7 +474,9 @@ int clw_batch_autotune(clw_batch* b, uint64_t q, const char* cache_path, clw_ban
  * sieving primes, with an additive checksum. Write alternately to two files;
  * read picks the valid file (right ranges, right sum) that covers more.
  * *result = 0 (no valid file: start from scratch) or 1 (resumed; continue
- * with windows from clw_batch_covered(); *which = 0 for a, 1 for b). */
+ * with windows from clw_batch_covered(); *which = 0 for a, 1 for b). Write
+ * first collects any finish in flight (clw_batch_finish_launch), so the file
+ * holds that range's results; read refuses a batch with a finish in flight. */
 int clw_batch_ckpt_write(clw_batch* b, const char* path);
 int clw_batch_ckpt_read(clw_batch* b, const char* path_a, const char* path_b, int* which, int* result);
 /* segments, sieve_integers, record/prime applications (device counters);

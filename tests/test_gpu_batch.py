@@ -323,6 +323,22 @@ def test_batch_checkpoint_with_a_finish_in_flight(dev, tmp_path):
             _same(b.results(r), ref[r])
 
 
+def test_batch_resume_refused_with_a_finish_in_flight(dev, tmp_path):
+    """Reading a checkpoint into a batch whose range has a finish in flight
+    is refused (the finish would hand back results of the replaced state);
+    the finish then collects the range's own results."""
+    ranges = MIX[2:4]
+    a, bp = str(tmp_path / "stateA.ckp"), str(tmp_path / "stateB.ckp")
+    with cw.Batch(dev, ranges) as b:
+        m0 = b.info[0].maxtarget
+        b.run(2, m0 + 1)
+        b.checkpoint(a)
+        b.finish_launch(0)
+        with pytest.raises(cw.ClwError):
+            b.resume(a, bp)
+        _same(b.finish_range(0), dev.wilson_range(*ranges[0]))
+
+
 def test_batch_checkpoint_resume(dev, tmp_path):
     """A batch checkpointed part-way (A, then B), rebuilt and resumed from
     the newer valid file, ends with the uninterrupted results; a corrupt B
